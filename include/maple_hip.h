@@ -1,0 +1,136 @@
+/* C ABI of the MI355X (gfx950) periodic inverse-homogenization library.
+ *
+ * Drop-in boundary for the reference's cell solve and homogenization
+ * (xing-yuu/MAPLE, src/maple/core).  All array arguments are plain pointers:
+ * "dev" arrays live in device memory (e.g. torch.Tensor.data_ptr() of a CUDA
+ * tensor), "host" arrays in host memory.  `stream` is a hipStream_t (NULL =
+ * the default stream); every kernel of a call is enqueued on it.  Every
+ * function returns 0 on success or a negative MAPLE_E* code; the message of
+ * the last failure on the calling thread is maple_last_error().  No function
+ * aborts or exits the process.
+ *
+ * Batches: a context holds `ncells` periodic cells, possibly of different
+ * resolution.  Per-element and per-node device arrays are the cells'
+ * arrays concatenated in cell order; inside a cell the element / node id is
+ * (x*ny + y)*nz + z (z fastest), as in build_unitcell_batch
+ * (main_IHD_v3.py:175-197).  Multi-case fields are stored plane by plane:
+ * value (case j, component c, node n) at [(j*nd + c)*Ntot + n], nd = 3
+ * (elastic, 6 cases) or 1 (thermal, 3 cases).
+ */
+#ifndef MAPLE_HIP_H
+#define MAPLE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MAPLE_OK 0
+#define MAPLE_EINVAL (-1)      /* bad argument */
+#define MAPLE_EDEVICE (-2)     /* no usable GPU / HIP runtime error */
+#define MAPLE_ENOMEM (-3)      /* device allocation failed */
+#define MAPLE_ENOCONV (-4)     /* CG hit maxit before rtol (results still written) */
+#define MAPLE_ENONFINITE (-5)  /* NaN / Inf in the input or the iteration */
+
+#define MAPLE_ELASTIC 1
+#define MAPLE_THERMAL 2
+
+#define MAPLE_PREC_FP64 0      /* plain method: every kernel in FP64 */
+#define MAPLE_PREC_MIXED 1     /* FP32 inner solve inside FP64 iterative refinement */
+
+typedef struct maple_ctx maple_ctx;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* maple_last_error(void);
+
+/* Library version string. */
+const char* maple_version(void);
+
+/* Number of HIP devices visible (0 without a GPU); never fails. */
+int maple_device_count(void);
+
+/* Element matrices of the reference (isotropic_elastic_tensor / hexahedron /
+ * hexahedron_scalar, _utils.py:39-196) for a cube of side 1/r, row-major FP64
+ * host arrays: Ke24 (24x24), Fe24 (24x6), X0_24 (24x6), Ke8 (8x8), Fe8 (8x3),
+ * X0_8 (8x3).  Pure host code. */
+int maple_hex8_matrices(int r, double E0, double nu, double kappa0, double* Ke24, double* Fe24, double* X0_24,
+                        double* Ke8, double* Fe8, double* X0_8);
+
+/* Create a context for `ncells` cells with node grids dims[3*i..3*i+2] on
+ * `device`, `nlevels` multigrid levels (every dim must be divisible by
+ * 2^(nlevels-1)), precision MAPLE_PREC_*.  The element matrices default to the
+ * reference's material (E0=1, nu=0.3, kappa0=1, model.py:18, 25).
+ * Replaces IHD.__init__ (model.py:13-32) and the per-call hierarchy build of
+ * GMGSolver.solve (EBE_GMG.py:790-849). */
+int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, int precision, maple_ctx** out);
+int maple_ctx_destroy(maple_ctx* ctx);
+
+/* Override the element matrices of one cell (host arrays, layouts as in
+ * maple_hex8_matrices; any pointer may be NULL to keep that physics). */
+int maple_set_element_matrices(maple_ctx* ctx, int cell, const double* Ke24, const double* Fe24, const double* X0_24,
+                               const double* Ke8, const double* Fe8, const double* X0_8);
+
+/* Solver knobs: Jacobi weight, sweeps per V-cycle side, and whether each call
+ * starts CG from the previous call's fields (1) or from zero (0, the reference,
+ * EBE_GMG.py:858). */
+int maple_set_solver(maple_ctx* ctx, double omega, int nsmooth, int warm_start);
+
+/* Homogenize: for every cell, solve the 6 elastic unit-strain (and/or 3 thermal
+ * unit-gradient) periodic cell problems K(rho) u = F(rho) by multigrid-
+ * preconditioned CG to ||r|| <= max(rtol ||f||, 1e-14 ||Fe||_F sqrt(E)),
+ * then reduce C_H = sum_e rho_e (X0-u_e)^T Ke (X0-u_e) and kappa_H likewise.
+ *   rho_dev   : Etot doubles (SIMP-scaled element moduli, > 0)
+ *   physics   : MAPLE_ELASTIC | MAPLE_THERMAL
+ *   CH_dev    : ncells*36 doubles (row-major 6x6 per cell) or NULL
+ *   kappa_dev : ncells*9 doubles or NULL
+ *   iters_host: ncells*9 ints (6 elastic then 3 thermal CG counts) or NULL
+ * Replaces IHD.TO_homo_3d / TO_homo_1d (model.py:98-159) with
+ * GMGSolver.solve / GMGSolverScalar.solve (EBE_GMG.py:758-885,
+ * EBE_GMG_scalar.py:617-725) and homogenized_batch_3d / _thermal
+ * (model.py:33-95).  Returns MAPLE_ENOCONV if any case hit maxit. */
+int maple_homogenize(maple_ctx* ctx, const double* rho_dev, int physics, double rtol, int maxit, double* CH_dev,
+                     double* kappa_dev, int* iters_host, void* stream);
+
+/* Element sensitivities of J from the last maple_homogenize:
+ *   dJdrho[e] = <G6_cell, S_e> + <G3_cell, T_e>,  S_e = (X0-u_e)^T Ke (X0-u_e)
+ * G6_dev (ncells*36) = dJ/dC_H, G3_dev (ncells*9) = dJ/dkappa (either NULL if
+ * that physics is unused).  The converged form of autograd.grad(J, rho)
+ * through TO_homo_3d / TO_homo_1d (main_IHD_v3.py:688). */
+int maple_sensitivity(maple_ctx* ctx, const double* G6_dev, const double* G3_dev, double* dJdrho_dev, void* stream);
+
+/* Periodic box mean of odd width k over each cell (avg_filter3d_x_periodic,
+ * main_IHD_v3.py:342-359).  in and out: Etot doubles, may not alias. */
+int maple_filter_box(maple_ctx* ctx, const double* in_dev, double* out_dev, int k, void* stream);
+
+/* OC update with nbisect bisection steps on lambda (oc_update,
+ * main_IHD_v3.py:362-380): volfrac/move/xmin are per-cell host arrays. */
+int maple_oc_update(maple_ctx* ctx, const double* x_dev, const double* dJdx_dev, const double* volfrac_host,
+                    const double* move_host, const double* xmin_host, int nbisect, double* xnew_dev, void* stream);
+
+/* RHS F(rho) of one physics (assemble_F_fast / assemble_F_fast_scalar,
+ * _utils.py:453-507), in the plane-major layout. */
+int maple_rhs(maple_ctx* ctx, int physics, const double* rho_dev, double* F_dev, void* stream);
+
+/* Fine-level operator v = K(rho) u for `ncase` fields (EBEOperator.matvec,
+ * EBE_GMG.py:412-430; EBEOperatorScalar.matvec, EBE_GMG_scalar.py:292-306). */
+int maple_apply_operator(maple_ctx* ctx, int physics, const double* rho_dev, int ncase, const double* u_dev,
+                         double* v_dev, void* stream);
+
+/* Copy the fields of the last solve (plane-major, 6*3 or 3*1 planes of Ntot)
+ * into u_dev, or seed the next solve's start from u_dev. */
+int maple_get_fields(maple_ctx* ctx, int physics, double* u_dev, void* stream);
+int maple_set_fields(maple_ctx* ctx, int physics, const double* u_dev, void* stream);
+
+/* Sizes: total elements (= nodes) at level 0 over all cells. */
+long long maple_num_elements(const maple_ctx* ctx);
+
+/* Kernel timing with HIP events on the call's stream around every launch of
+ * the elastic level-0 operator kernels (enable, then read: count of launches
+ * and their summed duration in ms; reading resets). */
+int maple_timing_enable(maple_ctx* ctx, int enable);
+int maple_timing_read(maple_ctx* ctx, int kind, long long* count, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAPLE_HIP_H */

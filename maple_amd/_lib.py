@@ -1,0 +1,97 @@
+"""ctypes binding of libmaple_hip.so (the C ABI declared in include/maple_hip.h).
+
+The library is built in-tree (``python -m maple_amd.build``) at
+``maple_amd/lib/libmaple_hip.so``.  Loading fails loudly when it is missing:
+there is no CPU fallback in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmaple_hip.so")
+
+MAPLE_OK = 0
+MAPLE_EINVAL = -1
+MAPLE_EDEVICE = -2
+MAPLE_ENOMEM = -3
+MAPLE_ENOCONV = -4
+MAPLE_ENONFINITE = -5
+MAPLE_ELASTIC = 1
+MAPLE_THERMAL = 2
+MAPLE_PREC_FP64 = 0
+MAPLE_PREC_MIXED = 1
+
+_P = C.c_void_p
+_D = C.c_double
+_I = C.c_int
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes); the complete exported surface of maple_hip.h
+SIGNATURES = {
+    "maple_last_error": (C.c_char_p, []),
+    "maple_version": (C.c_char_p, []),
+    "maple_device_count": (_I, []),
+    "maple_hex8_matrices": (_I, [_I, _D, _D, _D, _DP, _DP, _DP, _DP, _DP, _DP]),
+    "maple_ctx_create": (_I, [_I, _I, _IP, _I, _I, C.POINTER(_P)]),
+    "maple_ctx_destroy": (_I, [_P]),
+    "maple_set_element_matrices": (_I, [_P, _I, _DP, _DP, _DP, _DP, _DP, _DP]),
+    "maple_set_solver": (_I, [_P, _D, _I, _I]),
+    "maple_homogenize": (_I, [_P, _P, _I, _D, _I, _P, _P, _IP, _P]),
+    "maple_sensitivity": (_I, [_P, _P, _P, _P, _P]),
+    "maple_filter_box": (_I, [_P, _P, _P, _I, _P]),
+    "maple_oc_update": (_I, [_P, _P, _P, _DP, _DP, _DP, _I, _P, _P]),
+    "maple_rhs": (_I, [_P, _I, _P, _P, _P]),
+    "maple_apply_operator": (_I, [_P, _I, _P, _I, _P, _P, _P]),
+    "maple_get_fields": (_I, [_P, _I, _P, _P]),
+    "maple_set_fields": (_I, [_P, _I, _P, _P]),
+    "maple_num_elements": (C.c_longlong, [_P]),
+    "maple_timing_enable": (_I, [_P, _I]),
+    "maple_timing_read": (_I, [_P, _I, C.POINTER(C.c_longlong), _DP]),
+}
+
+_lib = None
+
+
+class MapleError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"maple_hip error {code}: {msg}")
+        self.code = code
+
+
+class NotConverged(MapleError):
+    pass
+
+
+def load(path: str = LIB_PATH):
+    """Load the library (once) and declare every signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m maple_amd.build` (hipcc, gfx950)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int):
+    if rc == MAPLE_OK:
+        return
+    msg = load().maple_last_error().decode(errors="replace")
+    if rc == MAPLE_EINVAL:
+        raise ValueError(f"maple_hip: {msg}")
+    if rc == MAPLE_ENOCONV:
+        raise NotConverged(rc, msg)
+    raise MapleError(rc, msg)
+
+
+def dptr(arr):
+    """ctypes double* of a C-contiguous float64 numpy array."""
+    return arr.ctypes.data_as(_DP)

@@ -1,0 +1,206 @@
+"""Python handle on a batch of periodic cells living on one MI355X.
+
+``CellBatch`` owns one library context (include/maple_hip.h): the multigrid
+hierarchy, CG vectors and element energies of every cell, allocated once.
+Inputs and outputs are torch tensors on the context's device (float64,
+contiguous); every call is enqueued on torch's current stream of that device.
+There is no CPU path: a missing library or a missing GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Iterable, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+PHYS = {"elastic": L.MAPLE_ELASTIC, "thermal": L.MAPLE_THERMAL}
+NCASE = {"elastic": 6, "thermal": 3}
+NDOF = {"elastic": 3, "thermal": 1}
+
+
+def hex8_matrices(r: int, E0: float = 1.0, nu: float = 0.3, kappa0: float = 1.0):
+    """Element matrices of the reference (_utils.py:39-196), from the library's host code.
+
+    Returns numpy float64 (Ke24, Fe24, X0_24, Ke8, Fe8, X0_8).
+    """
+    lib = L.load()
+    out = [np.zeros(s) for s in ((24, 24), (24, 6), (24, 6), (8, 8), (8, 3), (8, 3))]
+    L.check(lib.maple_hex8_matrices(int(r), float(E0), float(nu), float(kappa0), *[L.dptr(a) for a in out]))
+    return tuple(out)
+
+
+def max_levels(r: int, min_coarse: int = 4) -> int:
+    """Largest level count whose coarsest grid is at least min_coarse per side."""
+    n = 1
+    while r % (2 ** n) == 0 and r // (2 ** n) >= min_coarse:
+        n += 1
+    return n
+
+
+class CellBatch:
+    def __init__(self, dims: Sequence, nlevels: int = 4, device=None, precision: str = "fp64",
+                 omega: float = 0.6, nsmooth: int = 2, warm_start: bool = False):
+        self.lib = L.load()
+        dl = []
+        for d in dims:
+            if isinstance(d, (int, np.integer)):
+                dl.append((int(d),) * 3)
+            else:
+                dl.append(tuple(int(v) for v in d))
+        self.dims = dl
+        self.ncells = len(dl)
+        self.nlevels = int(nlevels)
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("CellBatch needs a CUDA (HIP) device; there is no CPU path")
+        prec = {"fp64": L.MAPLE_PREC_FP64, "mixed": L.MAPLE_PREC_MIXED}[precision]
+        self.precision = precision
+        flat = (C.c_int * (3 * self.ncells))(*[v for d in dl for v in d])
+        h = C.c_void_p()
+        L.check(self.lib.maple_ctx_create(self.device.index or 0, self.ncells, flat, self.nlevels, prec, C.byref(h)))
+        self._h = h
+        self.sizes = [int(np.prod(d)) for d in dl]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.nelem = int(self.offsets[-1])
+        self.generation = {"elastic": 0, "thermal": 0}
+        self.last_iters = None
+        self.set_solver(omega, nsmooth, warm_start)
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.maple_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- helpers ----------------------------------------------------------
+    def _stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _vec(self, t: torch.Tensor, n: int, name: str) -> torch.Tensor:
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{name} must be a torch.Tensor")
+        if t.device != self.device:
+            raise ValueError(f"{name} is on {t.device}, the batch is on {self.device}")
+        if t.numel() != n:
+            raise ValueError(f"{name} has {t.numel()} values, expected {n}")
+        return t.reshape(-1).to(torch.float64).contiguous()
+
+    def _empty(self, *shape):
+        return torch.empty(*shape, dtype=torch.float64, device=self.device)
+
+    # -- solver ------------------------------------------------------------
+    def set_solver(self, omega: float = 0.6, nsmooth: int = 2, warm_start: bool = False):
+        L.check(self.lib.maple_set_solver(self._h, float(omega), int(nsmooth), int(bool(warm_start))))
+
+    def set_element_matrices(self, cell: int, Ke24=None, Fe24=None, X0_24=None, Ke8=None, Fe8=None, X0_8=None):
+        def p(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+            return a
+        arrs = [p(a) for a in (Ke24, Fe24, X0_24, Ke8, Fe8, X0_8)]
+        ptrs = [L.dptr(a) if a is not None else None for a in arrs]
+        L.check(self.lib.maple_set_element_matrices(self._h, int(cell), *ptrs))
+
+    def homogenize(self, rho: torch.Tensor, physics: Iterable[str] = ("elastic", "thermal"),
+                   rtol: float = 1e-6, maxit: int = 500, strict: bool = True):
+        """Returns (C_H (ncells,6,6) or None, kappa (ncells,3,3) or None, iterations (ncells,9))."""
+        physics = tuple(physics)
+        rho = self._vec(rho, self.nelem, "rho")
+        mask = 0
+        for p in physics:
+            mask |= PHYS[p]
+        CH = self._empty(self.ncells, 6, 6) if "elastic" in physics else None
+        kap = self._empty(self.ncells, 3, 3) if "thermal" in physics else None
+        iters = np.zeros(self.ncells * 9, dtype=np.int32)
+        rc = self.lib.maple_homogenize(
+            self._h, C.c_void_p(rho.data_ptr()), mask, float(rtol), int(maxit),
+            C.c_void_p(CH.data_ptr()) if CH is not None else None,
+            C.c_void_p(kap.data_ptr()) if kap is not None else None,
+            iters.ctypes.data_as(L._IP), self._stream())
+        self.last_iters = iters.reshape(self.ncells, 9)
+        for p in physics:
+            self.generation[p] += 1
+        if rc != L.MAPLE_OK and (strict or rc != L.MAPLE_ENOCONV):
+            L.check(rc)
+        return CH, kap, self.last_iters
+
+    def sensitivity(self, G6: Optional[torch.Tensor] = None, G3: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """dJ/drho_e = <G6, S_e> + <G3, T_e> from the last homogenize (per-cell G)."""
+        g6 = self._vec(G6, 36 * self.ncells, "G6") if G6 is not None else None
+        g3 = self._vec(G3, 9 * self.ncells, "G3") if G3 is not None else None
+        out = self._empty(self.nelem)
+        L.check(self.lib.maple_sensitivity(
+            self._h, C.c_void_p(g6.data_ptr()) if g6 is not None else None,
+            C.c_void_p(g3.data_ptr()) if g3 is not None else None, C.c_void_p(out.data_ptr()), self._stream()))
+        return out
+
+    def filter(self, v: torch.Tensor, k: int = 3) -> torch.Tensor:
+        v = self._vec(v, self.nelem, "v")
+        out = self._empty(self.nelem)
+        L.check(self.lib.maple_filter_box(self._h, C.c_void_p(v.data_ptr()), C.c_void_p(out.data_ptr()), int(k),
+                                          self._stream()))
+        return out
+
+    def oc_update(self, x: torch.Tensor, dJdx: torch.Tensor, volfrac, move=0.2, xmin=1e-3, nbisect: int = 60):
+        x = self._vec(x, self.nelem, "x")
+        dJ = self._vec(dJdx, self.nelem, "dJdx")
+
+        def per_cell(v):
+            a = np.broadcast_to(np.asarray(v, dtype=np.float64), (self.ncells,)).copy()
+            return a
+        vf, mv, xm = per_cell(volfrac), per_cell(move), per_cell(xmin)
+        out = self._empty(self.nelem)
+        L.check(self.lib.maple_oc_update(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(dJ.data_ptr()), L.dptr(vf),
+                                         L.dptr(mv), L.dptr(xm), int(nbisect), C.c_void_p(out.data_ptr()),
+                                         self._stream()))
+        return out
+
+    def rhs(self, physics: str, rho: torch.Tensor) -> torch.Tensor:
+        rho = self._vec(rho, self.nelem, "rho")
+        F = self._empty(NCASE[physics], NDOF[physics], self.nelem)
+        L.check(self.lib.maple_rhs(self._h, PHYS[physics], C.c_void_p(rho.data_ptr()), C.c_void_p(F.data_ptr()),
+                                   self._stream()))
+        return F
+
+    def apply_operator(self, physics: str, rho: torch.Tensor, u: torch.Tensor) -> torch.Tensor:
+        """v = K(rho) u for u of shape (ncase, ndof, Etot), ncase <= the physics' case count."""
+        rho = self._vec(rho, self.nelem, "rho")
+        nd = NDOF[physics]
+        if u.dim() != 3 or u.shape[1] != nd or u.shape[2] != self.nelem:
+            raise ValueError(f"u must have shape (ncase, {nd}, {self.nelem})")
+        u = u.to(torch.float64).contiguous()
+        v = torch.empty_like(u)
+        L.check(self.lib.maple_apply_operator(self._h, PHYS[physics], C.c_void_p(rho.data_ptr()), int(u.shape[0]),
+                                              C.c_void_p(u.data_ptr()), C.c_void_p(v.data_ptr()), self._stream()))
+        return v
+
+    def fields(self, physics: str) -> torch.Tensor:
+        u = self._empty(NCASE[physics], NDOF[physics], self.nelem)
+        L.check(self.lib.maple_get_fields(self._h, PHYS[physics], C.c_void_p(u.data_ptr()), self._stream()))
+        return u
+
+    def set_fields(self, physics: str, u: torch.Tensor):
+        u = self._vec(u, NCASE[physics] * NDOF[physics] * self.nelem, "u")
+        L.check(self.lib.maple_set_fields(self._h, PHYS[physics], C.c_void_p(u.data_ptr()), self._stream()))
+
+    # -- timing -------------------------------------------------------------
+    def timing(self, enable: bool):
+        L.check(self.lib.maple_timing_enable(self._h, int(bool(enable))))
+
+    def timing_read(self, kind: int = 0):
+        cnt = C.c_longlong(0)
+        ms = C.c_double(0.0)
+        L.check(self.lib.maple_timing_read(self._h, int(kind), C.byref(cnt), C.byref(ms)))
+        return int(cnt.value), float(ms.value)

@@ -1,0 +1,678 @@
+// C ABI of the library: contexts, the multigrid hierarchy, the multi-RHS
+// preconditioned CG driver, homogenization, sensitivities, filter and OC.
+//
+// One context = one batch of periodic cells on one device.  Everything a
+// density field needs (stencils of every level, the dense coarsest inverse,
+// the CG vectors, the element energies) is allocated once at creation and
+// reused by every call; calls allocate nothing.  Each call runs on the
+// caller's stream; the only host synchronisations are the CG convergence
+// polls (one-iteration lag, so the device never waits for the host) and the
+// final read of the iteration counts.
+#include "../../include/maple_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "element.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                                  \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess) return fail(MAPLE_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define TRY(x)             \
+  do {                     \
+    int rc_ = (x);         \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+struct Level {
+  std::vector<LevelCell> h;
+  LevelCell* d = nullptr;
+  long long Ntot = 0;
+  int* dblk = nullptr;    // 256-node blocks
+  int* dfblk = nullptr;   // level 0: operator blocks (4 waves)
+  int* doblk = nullptr;   // level 0: 1024-element blocks (OC)
+  Launch node, fine, oc;
+};
+
+struct Phys {
+  int nd = 0, nc = 0;
+  double *Ke = nullptr, *Fe = nullptr, *X0 = nullptr, *G = nullptr;   // per cell on device
+  std::vector<double> hKe, hFe, hX0;                                   // per cell on host
+  std::vector<double*> S, dinv, X, B, R, T;
+  double *u = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *f = nullptr;
+  double *A = nullptr, *work = nullptr, *gamma = nullptr;
+  std::vector<DenseCell> hdc;
+  DenseCell* ddc = nullptr;
+  int maxn = 0, max_nt = 0;
+  double* Se = nullptr;
+  CaseScalars* sc = nullptr;
+  std::vector<CaseScalars> hsc;
+  int* done_d = nullptr;
+  int* done_h = nullptr;   // pinned, 2 slots
+  double* atol2 = nullptr;
+  bool have_fields = false, have_energy = false;
+};
+
+struct Timer {
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+};
+
+}  // namespace
+
+struct maple_ctx {
+  int device = 0, ncells = 0, nlevels = 0, prec = 0;
+  std::vector<int> dims;
+  std::vector<Level> lv;
+  Phys ph[2];
+  double* partial = nullptr;
+  double omega = 0.6;
+  int nsmooth = 2;
+  int warm = 0;
+  OCState* ocst = nullptr;
+  OCState* ocst_h = nullptr;
+  double* oc_a = nullptr;
+  hipEvent_t poll[2] = {nullptr, nullptr};
+  bool timing = false;
+  Timer timers[2];
+  std::vector<void*> allocs;
+  std::vector<void*> host_allocs;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(maple_ctx* c, T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(MAPLE_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+  }
+  c->allocs.push_back((void*)*p);
+  return 0;
+}
+
+int set_device(const maple_ctx* c) {
+  HIPCHK(hipSetDevice(c->device));
+  return 0;
+}
+
+int make_launch(maple_ctx* c, Level& L, int per_block, bool fine_waves_mode, int** dptr, Launch* out) {
+  std::vector<int> blk(c->ncells + 1, 0);
+  for (int i = 0; i < c->ncells; ++i) {
+    const LevelCell& lc = L.h[i];
+    long long n = fine_waves_mode ? (long long)fine_waves(lc) : (long long)lc.nx * lc.ny * lc.nz;
+    blk[i + 1] = blk[i] + (int)((n + per_block - 1) / per_block);
+  }
+  TRY(dalloc(c, dptr, blk.size()));
+  HIPCHK(hipMemcpy(*dptr, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice));
+  out->cells = L.d;
+  out->blk_start = *dptr;
+  out->ncells = c->ncells;
+  out->nblocks = blk.back();
+  out->Ntot = L.Ntot;
+  return 0;
+}
+
+int upload_phys_constants(maple_ctx* c, Phys& P) {
+  const int ne = 8 * P.nd;
+  std::vector<double> G((size_t)c->ncells * 8 * ne * ne);
+  for (int i = 0; i < c->ncells; ++i) child_galerkin(P.nd, &P.hKe[(size_t)i * ne * ne], &G[(size_t)i * 8 * ne * ne]);
+  HIPCHK(hipMemcpy(P.Ke, P.hKe.data(), P.hKe.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P.Fe, P.hFe.data(), P.hFe.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P.X0, P.hX0.data(), P.hX0.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P.G, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice));
+  // absolute residual floor per cell: (1e-14 ||Fe||_F sqrt(E))^2
+  std::vector<double> at(c->ncells);
+  for (int i = 0; i < c->ncells; ++i) {
+    double fn = 0.0;
+    for (int k = 0; k < ne * P.nc; ++k) fn += P.hFe[(size_t)i * ne * P.nc + k] * P.hFe[(size_t)i * ne * P.nc + k];
+    const double E = (double)c->lv[0].h[i].nx * c->lv[0].h[i].ny * c->lv[0].h[i].nz;
+    at[i] = 1e-28 * fn * E;
+  }
+  HIPCHK(hipMemcpy(P.atol2, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
+  P.nd = nd;
+  P.nc = nc;
+  const int ne = 8 * nd;
+  const int L = c->nlevels;
+  TRY(dalloc(c, &P.Ke, (size_t)c->ncells * ne * ne));
+  TRY(dalloc(c, &P.Fe, (size_t)c->ncells * ne * nc));
+  TRY(dalloc(c, &P.X0, (size_t)c->ncells * ne * nc));
+  TRY(dalloc(c, &P.G, (size_t)c->ncells * 8 * ne * ne));
+  TRY(dalloc(c, &P.atol2, (size_t)c->ncells));
+  P.S.assign(L, nullptr);
+  P.dinv.assign(L, nullptr);
+  P.X.assign(L, nullptr);
+  P.B.assign(L, nullptr);
+  P.R.assign(L, nullptr);
+  P.T.assign(L, nullptr);
+  for (int l = 0; l < L; ++l) {
+    const long long N = c->lv[l].Ntot;
+    const size_t nv = (size_t)nc * nd * N;
+    if (l >= 1) TRY(dalloc(c, &P.S[l], (size_t)27 * nd * nd * N));
+    if (l < L - 1) {
+      TRY(dalloc(c, &P.dinv[l], (size_t)nd * N));
+      TRY(dalloc(c, &P.R[l], nv));
+      TRY(dalloc(c, &P.T[l], nv));
+    }
+    TRY(dalloc(c, &P.X[l], nv));
+    if (l >= 1) TRY(dalloc(c, &P.B[l], nv));
+  }
+  const size_t n0 = (size_t)nc * nd * c->lv[0].Ntot;
+  TRY(dalloc(c, &P.u, n0));
+  TRY(dalloc(c, &P.r, n0));
+  TRY(dalloc(c, &P.p, n0));
+  TRY(dalloc(c, &P.q, n0));
+  TRY(dalloc(c, &P.f, n0));
+  HIPCHK(hipMemset(P.u, 0, n0 * sizeof(double)));
+  // dense coarsest level
+  const Level& LC = c->lv[L - 1];
+  P.hdc.resize(c->ncells);
+  long long aoff = 0;
+  for (int i = 0; i < c->ncells; ++i) {
+    const LevelCell& lc = LC.h[i];
+    DenseCell dc;
+    dc.n = nd * lc.nx * lc.ny * lc.nz;
+    dc.nt = (dc.n + 63) / 64;
+    dc.aoff = aoff;
+    dc.voff = lc.off;
+    aoff += (long long)dc.n * dc.n;
+    P.hdc[i] = dc;
+    P.maxn = std::max(P.maxn, dc.n);
+    P.max_nt = std::max(P.max_nt, dc.nt);
+  }
+  if (P.maxn > 12288) return fail(MAPLE_EINVAL, "coarsest level too large for the dense solve (" + std::to_string(P.maxn) + " unknowns); use more levels");
+  TRY(dalloc(c, &P.A, (size_t)aoff));
+  TRY(dalloc(c, &P.work, (size_t)c->ncells * 64 * 64));
+  TRY(dalloc(c, &P.gamma, (size_t)c->ncells));
+  TRY(dalloc(c, &P.ddc, (size_t)c->ncells));
+  HIPCHK(hipMemcpy(P.ddc, P.hdc.data(), P.hdc.size() * sizeof(DenseCell), hipMemcpyHostToDevice));
+  TRY(dalloc(c, &P.Se, (size_t)nc * (nc + 1) / 2 * c->lv[0].Ntot));
+  TRY(dalloc(c, &P.sc, (size_t)c->ncells * nc));
+  TRY(dalloc(c, &P.done_d, (size_t)c->ncells * nc));
+  P.hsc.resize((size_t)c->ncells * nc);
+  void* hp = nullptr;
+  HIPCHK(hipHostMalloc(&hp, sizeof(int) * 2 * c->ncells * nc, hipHostMallocDefault));
+  c->host_allocs.push_back(hp);
+  P.done_h = (int*)hp;
+  return 0;
+}
+
+// ---- solver pieces ---------------------------------------------------------
+
+void timed_begin(maple_ctx* c, int kind, hipStream_t s) {
+  if (!c->timing) return;
+  Timer& t = c->timers[kind];
+  if (t.used + 2 > t.ev.size()) {
+    for (int k = 0; k < 256; ++k) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      t.ev.push_back(e);
+    }
+  }
+  (void)hipEventRecord(t.ev[t.used], s);
+}
+
+void timed_end(maple_ctx* c, int kind, hipStream_t s) {
+  if (!c->timing) return;
+  Timer& t = c->timers[kind];
+  if (t.used + 2 > t.ev.size()) return;
+  (void)hipEventRecord(t.ev[t.used + 1], s);
+  t.used += 2;
+}
+
+void apply(maple_ctx* c, Phys& P, int l, int mode, const double* rho, const double* u, const double* f, double* out,
+           double* partial, hipStream_t s) {
+  if (l == 0) {
+    const int kind = P.nd == 3 ? 0 : 1;
+    timed_begin(c, kind, s);
+    if (P.nd == 3)
+      fine_apply<3, double>(c->lv[0].fine, mode, P.nc, P.Ke, rho, u, f, P.dinv[0], c->omega, out, partial, s);
+    else
+      fine_apply<1, double>(c->lv[0].fine, mode, P.nc, P.Ke, rho, u, f, P.dinv[0], c->omega, out, partial, s);
+    timed_end(c, kind, s);
+  } else {
+    if (P.nd == 3)
+      stencil_apply<3, double>(c->lv[l].node, mode, P.nc, P.S[l], u, f, P.dinv[l], c->omega, out, nullptr, s);
+    else
+      stencil_apply<1, double>(c->lv[l].node, mode, P.nc, P.S[l], u, f, P.dinv[l], c->omega, out, nullptr, s);
+  }
+}
+
+void setup_hierarchy(maple_ctx* c, Phys& P, const double* rho, hipStream_t s) {
+  const int L = c->nlevels;
+  fine_dinv<double>(c->lv[0].node, P.nd, P.Ke, rho, P.dinv[0], s);
+  stencil_level1<double>(c->lv[1].node, c->lv[0].node, P.nd, P.G, rho, P.S[1], s);
+  for (int l = 2; l < L; ++l) stencil_galerkin<double>(c->lv[l].node, c->lv[l - 1].node, P.nd, P.S[l - 1], P.S[l], s);
+  for (int l = 1; l < L - 1; ++l) stencil_dinv<double>(c->lv[l].node, P.nd, P.S[l], P.dinv[l], s);
+  dense_assemble_g<double>(c->lv[L - 1].node, P.ddc, P.maxn, P.nd, P.S[L - 1], P.A, P.gamma, s);
+  dense_inverse(P.ddc, c->ncells, P.max_nt, P.A, P.work, s);
+}
+
+// z = M b at level l (symmetric V-cycle, zero start); returns the buffer holding the result.
+double* vcycle(maple_ctx* c, Phys& P, int l, const double* b, const double* rho, hipStream_t s) {
+  const int L = c->nlevels;
+  if (l == L - 1) {
+    dense_apply<double>(c->lv[l].node, P.ddc, P.maxn, P.nd, P.nc, P.A, b, P.X[l], s);
+    return P.X[l];
+  }
+  double* x = P.X[l];
+  double* t = P.T[l];
+  vec_scale_dinv<double>(c->lv[l].node, P.nd, P.nc, c->omega, P.dinv[l], b, x, s);
+  for (int k = 1; k < c->nsmooth; ++k) {
+    apply(c, P, l, 3, rho, x, b, t, nullptr, s);
+    std::swap(x, t);
+  }
+  apply(c, P, l, 2, rho, x, b, P.R[l], nullptr, s);
+  restrict_pt<double>(c->lv[l + 1].node, c->lv[l].node, P.nd, P.nc, P.R[l], P.B[l + 1], s);
+  double* e = vcycle(c, P, l + 1, P.B[l + 1], rho, s);
+  prolong_add<double>(c->lv[l].node, c->lv[l + 1].node, P.nd, P.nc, e, x, s);
+  for (int k = 0; k < c->nsmooth; ++k) {
+    apply(c, P, l, 3, rho, x, b, t, nullptr, s);
+    std::swap(x, t);
+  }
+  return x;
+}
+
+int poll_done(maple_ctx* c, Phys& P, int slot, bool* all) {
+  HIPCHK(hipEventSynchronize(c->poll[slot]));
+  const int n = c->ncells * P.nc;
+  bool a = true;
+  for (int i = 0; i < n; ++i) a = a && P.done_h[slot * n + i];
+  *all = a;
+  return 0;
+}
+
+int enqueue_poll(maple_ctx* c, Phys& P, int slot, hipStream_t s) {
+  const int n = c->ncells * P.nc;
+  HIPCHK(hipMemcpyAsync(P.done_h + slot * n, P.done_d, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(c->poll[slot], s));
+  return 0;
+}
+
+int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStream_t s) {
+  Level& L0 = c->lv[0];
+  const long long n0 = (long long)P.nc * P.nd * L0.Ntot;
+  fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
+  vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, c->partial, s);
+  reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_FF, s);
+  if (c->warm && P.have_fields) {
+    apply(c, P, 0, 2, rho, P.u, P.f, P.r, nullptr, s);
+  } else {
+    vec_zero<double>(n0, P.u, s);
+    vec_copy<double, double>(n0, P.f, P.r, s);
+  }
+  vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, c->partial, s);
+  reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
+  double* z = vcycle(c, P, 0, P.r, rho, s);
+  vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, c->partial, s);
+  reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RZ_NEW, s);
+  pcg_init_scalars(c->ncells, P.nc, rtol, P.atol2, P.sc, P.done_d, s);
+  vec_copy<double, double>(n0, z, P.p, s);
+  TRY(enqueue_poll(c, P, 0, s));
+  bool all = false;
+  TRY(poll_done(c, P, 0, &all));
+  for (int it = 0; it < maxit && !all; ++it) {
+    apply(c, P, 0, 1, rho, P.p, nullptr, P.q, c->partial, s);
+    reduce_partials(L0.fine.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_PQ, s);
+    pcg_alpha(c->ncells, P.nc, P.sc, s);
+    pcg_update_ur(L0.node, P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, c->partial, s);
+    reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
+    pcg_check(c->ncells, P.nc, P.sc, P.done_d, s);
+    const int slot = (it + 1) & 1;
+    TRY(enqueue_poll(c, P, slot, s));
+    if (it + 1 < maxit) {
+      z = vcycle(c, P, 0, P.r, rho, s);
+      vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, c->partial, s);
+      reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RZ_NEW, s);
+      pcg_beta(c->ncells, P.nc, P.sc, s);
+      pcg_update_p(L0.node, P.nd, P.nc, P.sc, z, P.p, s);
+    }
+    // the device is already working on the next iteration while we look at this one
+    TRY(poll_done(c, P, slot, &all));
+  }
+  P.have_fields = true;
+  HIPCHK(hipMemcpyAsync(P.hsc.data(), P.sc, sizeof(CaseScalars) * P.hsc.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (const CaseScalars& cs : P.hsc)
+    if (!(cs.rr == cs.rr) || !std::isfinite(cs.ff)) return fail(MAPLE_ENONFINITE, "non-finite residual in the cell solve");
+  return all ? 0 : MAPLE_ENOCONV;
+}
+
+int check_ctx(const maple_ctx* c) {
+  if (!c) return fail(MAPLE_EINVAL, "null context");
+  return set_device(c);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char* maple_last_error(void) { return g_err.c_str(); }
+
+const char* maple_version(void) { return "maple_hip 0.1 gfx950"; }
+
+int maple_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int maple_hex8_matrices(int r, double E0, double nu, double kappa0, double* Ke24, double* Fe24, double* X0_24,
+                        double* Ke8, double* Fe8, double* X0_8) {
+  if (r <= 0) return fail(MAPLE_EINVAL, "r must be positive");
+  if (!(nu > -1.0 && nu < 0.5)) return fail(MAPLE_EINVAL, "nu must lie in (-1, 0.5)");
+  if (Ke24 && Fe24 && X0_24) hex8_elastic(r, E0, nu, Ke24, Fe24, X0_24);
+  if (Ke8 && Fe8 && X0_8) hex8_thermal(r, kappa0, Ke8, Fe8, X0_8);
+  return 0;
+}
+
+int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, int precision, maple_ctx** out) {
+  if (!out) return fail(MAPLE_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (ncells <= 0 || !dims_host) return fail(MAPLE_EINVAL, "need at least one cell");
+  if (nlevels < 2 || nlevels > 12) return fail(MAPLE_EINVAL, "nlevels must be in [2, 12]");
+  if (precision != MAPLE_PREC_FP64) return fail(MAPLE_EINVAL, "unknown precision");
+  for (int i = 0; i < 3 * ncells; ++i) {
+    const int d = dims_host[i];
+    if (d <= 0 || d % (1 << (nlevels - 1)) != 0)
+      return fail(MAPLE_EINVAL, "every cell dimension must be a positive multiple of 2^(nlevels-1)");
+  }
+  for (int i = 0; i < ncells; ++i)
+    if (dims_host[3 * i] != dims_host[3 * i + 1] || dims_host[3 * i] != dims_host[3 * i + 2])
+      return fail(MAPLE_EINVAL, "cells must be cubic (the element side is 1/r)");
+  int ndev = maple_device_count();
+  if (ndev <= 0) return fail(MAPLE_EDEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(MAPLE_EINVAL, "device index out of range");
+  maple_ctx* c = new maple_ctx();
+  c->device = device;
+  c->ncells = ncells;
+  c->nlevels = nlevels;
+  c->prec = precision;
+  c->dims.assign(dims_host, dims_host + 3 * ncells);
+  int rc = set_device(c);
+  auto bail = [&](int code) {
+    std::string keep = g_err;
+    maple_ctx_destroy(c);
+    g_err = keep;
+    return code;
+  };
+  if (rc) return bail(rc);
+  c->lv.resize(nlevels);
+  for (int l = 0; l < nlevels; ++l) {
+    Level& L = c->lv[l];
+    L.h.resize(ncells);
+    long long off = 0;
+    for (int i = 0; i < ncells; ++i) {
+      LevelCell lc;
+      lc.nx = c->dims[3 * i] >> l;
+      lc.ny = c->dims[3 * i + 1] >> l;
+      lc.nz = c->dims[3 * i + 2] >> l;
+      lc.pad_ = 0;
+      lc.off = off;
+      off += (long long)lc.nx * lc.ny * lc.nz;
+      L.h[i] = lc;
+    }
+    L.Ntot = off;
+    if ((rc = dalloc(c, &L.d, ncells))) return bail(rc);
+    if (hipMemcpy(L.d, L.h.data(), sizeof(LevelCell) * ncells, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(MAPLE_EDEVICE, "upload of the cell table failed"));
+    if ((rc = make_launch(c, L, 256, false, &L.dblk, &L.node))) return bail(rc);
+    if (l == 0) {
+      if ((rc = make_launch(c, L, 4, true, &L.dfblk, &L.fine))) return bail(rc);
+      if ((rc = make_launch(c, L, 1024, false, &L.doblk, &L.oc))) return bail(rc);
+    }
+  }
+  // partial-sum scratch: 64 doubles per block of the largest launch
+  int maxb = std::max(c->lv[0].node.nblocks, c->lv[0].fine.nblocks);
+  if ((rc = dalloc(c, &c->partial, (size_t)maxb * 64))) return bail(rc);
+  if ((rc = alloc_phys(c, c->ph[0], 3, 6))) return bail(rc);
+  if ((rc = alloc_phys(c, c->ph[1], 1, 3))) return bail(rc);
+  for (int p = 0; p < 2; ++p) {
+    Phys& P = c->ph[p];
+    const int ne = 8 * P.nd;
+    P.hKe.resize((size_t)ncells * ne * ne);
+    P.hFe.resize((size_t)ncells * ne * P.nc);
+    P.hX0.resize((size_t)ncells * ne * P.nc);
+  }
+  for (int i = 0; i < ncells; ++i) {
+    const int r = c->dims[3 * i];
+    hex8_elastic(r, 1.0, 0.3, &c->ph[0].hKe[(size_t)i * 576], &c->ph[0].hFe[(size_t)i * 144],
+                 &c->ph[0].hX0[(size_t)i * 144]);
+    hex8_thermal(r, 1.0, &c->ph[1].hKe[(size_t)i * 64], &c->ph[1].hFe[(size_t)i * 24], &c->ph[1].hX0[(size_t)i * 24]);
+  }
+  for (int p = 0; p < 2; ++p)
+    if ((rc = upload_phys_constants(c, c->ph[p]))) return bail(rc);
+  if ((rc = dalloc(c, &c->ocst, ncells))) return bail(rc);
+  if ((rc = dalloc(c, &c->oc_a, (size_t)c->lv[0].Ntot))) return bail(rc);
+  void* hp = nullptr;
+  if (hipHostMalloc(&hp, sizeof(OCState) * ncells, hipHostMallocDefault) != hipSuccess)
+    return bail(fail(MAPLE_ENOMEM, "pinned host allocation failed"));
+  c->host_allocs.push_back(hp);
+  c->ocst_h = (OCState*)hp;
+  for (int k = 0; k < 2; ++k)
+    if (hipEventCreateWithFlags(&c->poll[k], hipEventDisableTiming) != hipSuccess)
+      return bail(fail(MAPLE_EDEVICE, "event creation failed"));
+  *out = c;
+  return 0;
+}
+
+int maple_ctx_destroy(maple_ctx* c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();
+  for (void* p : c->allocs) (void)hipFree(p);
+  for (void* p : c->host_allocs) (void)hipHostFree(p);
+  for (int k = 0; k < 2; ++k)
+    if (c->poll[k]) (void)hipEventDestroy(c->poll[k]);
+  for (Timer& t : c->timers)
+    for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
+  delete c;
+  return 0;
+}
+
+int maple_set_element_matrices(maple_ctx* c, int cell, const double* Ke24, const double* Fe24, const double* X0_24,
+                               const double* Ke8, const double* Fe8, const double* X0_8) {
+  TRY(check_ctx(c));
+  if (cell < 0 || cell >= c->ncells) return fail(MAPLE_EINVAL, "cell index out of range");
+  const double* src[2][3] = {{Ke24, Fe24, X0_24}, {Ke8, Fe8, X0_8}};
+  for (int p = 0; p < 2; ++p) {
+    Phys& P = c->ph[p];
+    const int ne = 8 * P.nd;
+    if (!src[p][0] && !src[p][1] && !src[p][2]) continue;
+    if (!src[p][0] || !src[p][1] || !src[p][2]) return fail(MAPLE_EINVAL, "give Ke, Fe and X0 together");
+    std::memcpy(&P.hKe[(size_t)cell * ne * ne], src[p][0], sizeof(double) * ne * ne);
+    std::memcpy(&P.hFe[(size_t)cell * ne * P.nc], src[p][1], sizeof(double) * ne * P.nc);
+    std::memcpy(&P.hX0[(size_t)cell * ne * P.nc], src[p][2], sizeof(double) * ne * P.nc);
+    TRY(upload_phys_constants(c, P));
+  }
+  return 0;
+}
+
+int maple_set_solver(maple_ctx* c, double omega, int nsmooth, int warm_start) {
+  if (!c) return fail(MAPLE_EINVAL, "null context");
+  if (!(omega > 0.0 && omega < 2.0)) return fail(MAPLE_EINVAL, "omega must lie in (0, 2)");
+  if (nsmooth < 1 || nsmooth > 16) return fail(MAPLE_EINVAL, "nsmooth must lie in [1, 16]");
+  c->omega = omega;
+  c->nsmooth = nsmooth;
+  c->warm = warm_start ? 1 : 0;
+  return 0;
+}
+
+int maple_homogenize(maple_ctx* c, const double* rho, int physics, double rtol, int maxit, double* CH,
+                     double* kappa, int* iters_host, void* stream) {
+  TRY(check_ctx(c));
+  if (!rho) return fail(MAPLE_EINVAL, "rho is NULL");
+  if (!(physics & (MAPLE_ELASTIC | MAPLE_THERMAL))) return fail(MAPLE_EINVAL, "no physics selected");
+  if (!(rtol > 0.0) || maxit < 1) return fail(MAPLE_EINVAL, "need rtol > 0 and maxit >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  int status = 0;
+  for (int p = 0; p < 2; ++p) {
+    if (!(physics & (p == 0 ? MAPLE_ELASTIC : MAPLE_THERMAL))) continue;
+    Phys& P = c->ph[p];
+    double* out = p == 0 ? CH : kappa;
+    setup_hierarchy(c, P, rho, s);
+    HIPCHK(hipGetLastError());
+    int rc = pcg(c, P, rho, rtol, maxit, s);
+    if (rc == MAPLE_ENOCONV) {
+      status = rc;
+      g_err = std::string(p == 0 ? "elastic" : "thermal") + " CG reached maxit before rtol";
+    } else if (rc) {
+      return rc;
+    }
+    element_energy(c->lv[0].node, P.nd, P.nc, P.Ke, P.X0, rho, P.u, P.Se, c->partial, s);
+    if (out) reduce_energy(c->lv[0].node.blk_start, c->ncells, P.nc * (P.nc + 1) / 2, c->partial, P.nc, out, s);
+    HIPCHK(hipGetLastError());
+    P.have_energy = true;
+    if (iters_host)
+      for (int i = 0; i < c->ncells; ++i)
+        for (int j = 0; j < P.nc; ++j) iters_host[i * 9 + (p == 0 ? 0 : 6) + j] = P.hsc[i * P.nc + j].iters;
+  }
+  return status;
+}
+
+int maple_sensitivity(maple_ctx* c, const double* G6, const double* G3, double* out, void* stream) {
+  TRY(check_ctx(c));
+  if (!out) return fail(MAPLE_EINVAL, "output is NULL");
+  if (G6 && !c->ph[0].have_energy) return fail(MAPLE_EINVAL, "no elastic solve to differentiate");
+  if (G3 && !c->ph[1].have_energy) return fail(MAPLE_EINVAL, "no thermal solve to differentiate");
+  if (!G6 && !G3) return fail(MAPLE_EINVAL, "give G6 and/or G3");
+  sensitivity(c->lv[0].node, G6 ? 6 : 0, c->ph[0].Se, G6, G3 ? 3 : 0, c->ph[1].Se, G3, out, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int maple_filter_box(maple_ctx* c, const double* in, double* out, int k, void* stream) {
+  TRY(check_ctx(c));
+  if (k <= 0 || k % 2 == 0) return fail(MAPLE_EINVAL, "filter size must be positive and odd");
+  if (!in || !out || in == out) return fail(MAPLE_EINVAL, "need distinct input and output");
+  box_filter(c->lv[0].node, k, in, out, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int maple_oc_update(maple_ctx* c, const double* x, const double* dJ, const double* volfrac, const double* move,
+                    const double* xmin, int nbisect, double* xnew, void* stream) {
+  TRY(check_ctx(c));
+  if (!x || !dJ || !xnew || !volfrac || !move || !xmin) return fail(MAPLE_EINVAL, "null argument");
+  if (nbisect < 1 || nbisect > 200) return fail(MAPLE_EINVAL, "nbisect must lie in [1, 200]");
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(hipStreamSynchronize(s));   // the pinned staging buffer may still feed a previous call
+  for (int i = 0; i < c->ncells; ++i) {
+    c->ocst_h[i].volfrac = volfrac[i];
+    c->ocst_h[i].move = move[i];
+    c->ocst_h[i].xmin = xmin[i];
+  }
+  HIPCHK(hipMemcpyAsync(c->ocst, c->ocst_h, sizeof(OCState) * c->ncells, hipMemcpyHostToDevice, s));
+  Level& L0 = c->lv[0];
+  oc_prepare(L0.node, x, dJ, c->oc_a, s);
+  oc_start(c->ncells, c->ocst, s);
+  for (int done = 0; done < nbisect; done += 6) {
+    const int depth = std::min(6, nbisect - done);
+    oc_pass(L0.oc, c->ocst, x, c->oc_a, c->partial, s);
+    oc_walk(L0.oc.blk_start, L0.d, c->ncells, depth, c->partial, c->ocst, s);
+  }
+  oc_final(L0.node, c->ocst, x, c->oc_a, xnew, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int maple_rhs(maple_ctx* c, int physics, const double* rho, double* F, void* stream) {
+  TRY(check_ctx(c));
+  if (physics != MAPLE_ELASTIC && physics != MAPLE_THERMAL) return fail(MAPLE_EINVAL, "physics must be one of ELASTIC, THERMAL");
+  if (!rho || !F) return fail(MAPLE_EINVAL, "null argument");
+  Phys& P = c->ph[physics == MAPLE_ELASTIC ? 0 : 1];
+  fine_rhs(c->lv[0].node, P.nd, P.nc, P.Fe, rho, F, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int maple_apply_operator(maple_ctx* c, int physics, const double* rho, int ncase, const double* u, double* v,
+                         void* stream) {
+  TRY(check_ctx(c));
+  if (physics != MAPLE_ELASTIC && physics != MAPLE_THERMAL) return fail(MAPLE_EINVAL, "physics must be one of ELASTIC, THERMAL");
+  if (!rho || !u || !v || u == v) return fail(MAPLE_EINVAL, "need rho and distinct u, v");
+  Phys& P = c->ph[physics == MAPLE_ELASTIC ? 0 : 1];
+  if (ncase < 1 || ncase > P.nc) return fail(MAPLE_EINVAL, "ncase out of range");
+  hipStream_t s = (hipStream_t)stream;
+  const int keep = P.nc;
+  P.nc = ncase;
+  apply(c, P, 0, 0, rho, u, nullptr, v, nullptr, s);
+  P.nc = keep;
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int maple_get_fields(maple_ctx* c, int physics, double* u, void* stream) {
+  TRY(check_ctx(c));
+  if (physics != MAPLE_ELASTIC && physics != MAPLE_THERMAL) return fail(MAPLE_EINVAL, "physics must be one of ELASTIC, THERMAL");
+  Phys& P = c->ph[physics == MAPLE_ELASTIC ? 0 : 1];
+  if (!u) return fail(MAPLE_EINVAL, "null output");
+  HIPCHK(hipMemcpyAsync(u, P.u, sizeof(double) * P.nc * P.nd * c->lv[0].Ntot, hipMemcpyDeviceToDevice,
+                        (hipStream_t)stream));
+  return 0;
+}
+
+int maple_set_fields(maple_ctx* c, int physics, const double* u, void* stream) {
+  TRY(check_ctx(c));
+  if (physics != MAPLE_ELASTIC && physics != MAPLE_THERMAL) return fail(MAPLE_EINVAL, "physics must be one of ELASTIC, THERMAL");
+  Phys& P = c->ph[physics == MAPLE_ELASTIC ? 0 : 1];
+  if (!u) return fail(MAPLE_EINVAL, "null input");
+  HIPCHK(hipMemcpyAsync(P.u, u, sizeof(double) * P.nc * P.nd * c->lv[0].Ntot, hipMemcpyDeviceToDevice,
+                        (hipStream_t)stream));
+  P.have_fields = true;
+  return 0;
+}
+
+long long maple_num_elements(const maple_ctx* c) { return c ? c->lv[0].Ntot : 0; }
+
+int maple_timing_enable(maple_ctx* c, int enable) {
+  if (!c) return fail(MAPLE_EINVAL, "null context");
+  c->timing = enable != 0;
+  return 0;
+}
+
+int maple_timing_read(maple_ctx* c, int kind, long long* count, double* total_ms) {
+  TRY(check_ctx(c));
+  if (kind < 0 || kind > 1 || !count || !total_ms) return fail(MAPLE_EINVAL, "bad timing query");
+  Timer& t = c->timers[kind];
+  double ms = 0.0;
+  if (t.used > 0) HIPCHK(hipEventSynchronize(t.ev[t.used - 1]));
+  for (size_t k = 0; k + 1 < t.used; k += 2) {
+    float m = 0.0f;
+    HIPCHK(hipEventElapsedTime(&m, t.ev[k], t.ev[k + 1]));
+    ms += m;
+  }
+  *count = (long long)(t.used / 2);
+  *total_ms = ms;
+  t.used = 0;
+  return 0;
+}
+
+}  // extern "C"

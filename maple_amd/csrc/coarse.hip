@@ -1,0 +1,327 @@
+// Coarse multigrid levels on assembled 27-point block stencils.
+//
+// The reference keeps per-element coarse matrices and re-projects them with
+// the local trilinear operator (sparse_coarsen_ke, EBE_GMG.py:226-337), and
+// moves residuals with the periodic trilinear P / P^T (EBE_GMG.py:38-79,
+// 531-647).  Here every coarse level is stored as its assembled operator, one
+// ND x ND block per (node, neighbour offset), plane-major so that a wavefront
+// reads 64 consecutive nodes of one entry.  The operator is the same
+// Galerkin P^T K P: level 1 is built straight from the voxel densities with
+// the 8 constant child matrices G_c = P_c^T Ke P_c, deeper levels by the
+// Galerkin triple product on stencils.
+#include "kernels.h"
+
+namespace {
+
+__device__ __forceinline__ bool node_of(const LevelCell* cells, const int* blk_start, int ncells, int bid,
+                                        LevelCell& c, int& ci, int& x, int& y, int& z, long long& gid) {
+  ci = find_cell(blk_start, ncells, bid);
+  c = cells[ci];
+  const long long n = (long long)(bid - blk_start[ci]) * 256 + threadIdx.x;
+  const long long nn = (long long)c.nx * c.ny * c.nz;
+  if (n >= nn) return false;
+  z = (int)(n % c.nz);
+  const long long t = n / c.nz;
+  y = (int)(t % c.ny);
+  x = (int)(t / c.ny);
+  gid = c.off + n;
+  return true;
+}
+
+__device__ __forceinline__ long long nid(const LevelCell& c, int x, int y, int z) {
+  return c.off + ((long long)x * c.ny + y) * c.nz + z;
+}
+
+__device__ __forceinline__ double w1(int t) { return t == 0 ? 1.0 : ((t == 1 || t == -1) ? 0.5 : 0.0); }
+
+}  // namespace
+
+// MODE: 0 = K u, 2 = f - K u, 3 = u + omega Dinv (f - K u)
+template <int ND, typename T, int MODE>
+__global__ __launch_bounds__(256) void k_stencil_apply(const LevelCell* __restrict__ cells,
+                                                       const int* __restrict__ blk_start, int ncells,
+                                                       long long Ntot, int ncase, const T* __restrict__ S,
+                                                       const T* __restrict__ u, const T* __restrict__ f,
+                                                       const T* __restrict__ dinv, T omega,
+                                                       T* __restrict__ out) {
+  LevelCell c; int ci, x, y, z; long long gid;
+  if (!node_of(cells, blk_start, ncells, blockIdx.x, c, ci, x, y, z, gid)) return;
+  T acc[MAPLE_MAX_CASES][ND];
+#pragma unroll
+  for (int j = 0; j < MAPLE_MAX_CASES; ++j)
+#pragma unroll
+    for (int i = 0; i < ND; ++i) acc[j][i] = T(0);
+  for (int dx = -1; dx <= 1; ++dx) {
+    const int X = wrap(x + dx, c.nx);
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int Y = wrap(y + dy, c.ny);
+#pragma unroll
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int Z = wrap(z + dz, c.nz);
+        const int d = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+        const long long nb = nid(c, X, Y, Z);
+        T blk[ND][ND];
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+#pragma unroll
+          for (int e = 0; e < ND; ++e) blk[i][e] = S[(long long)(d * ND * ND + i * ND + e) * Ntot + gid];
+#pragma unroll
+        for (int j = 0; j < MAPLE_MAX_CASES; ++j) {
+          if (j < ncase) {
+            T ue[ND];
+#pragma unroll
+            for (int e = 0; e < ND; ++e) ue[e] = u[(long long)(j * ND + e) * Ntot + nb];
+#pragma unroll
+            for (int i = 0; i < ND; ++i)
+#pragma unroll
+              for (int e = 0; e < ND; ++e) acc[j][i] = fma(blk[i][e], ue[e], acc[j][i]);
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < MAPLE_MAX_CASES; ++j) {
+    if (j < ncase) {
+#pragma unroll
+      for (int i = 0; i < ND; ++i) {
+        const long long idx = (long long)(j * ND + i) * Ntot + gid;
+        if constexpr (MODE == 0) out[idx] = acc[j][i];
+        else if constexpr (MODE == 2) out[idx] = f[idx] - acc[j][i];
+        else out[idx] = u[idx] + omega * dinv[(long long)i * Ntot + gid] * (f[idx] - acc[j][i]);
+      }
+    }
+  }
+}
+
+template <int ND, typename T>
+void stencil_apply(const Launch& L, int mode, int ncase, const T* S, const T* u, const T* f, const T* dinv,
+                   T omega, T* out, double* /*partial*/, hipStream_t s) {
+  if (mode == 0) k_stencil_apply<ND, T, 0><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, S, u, f, dinv, omega, out);
+  else if (mode == 2) k_stencil_apply<ND, T, 2><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, S, u, f, dinv, omega, out);
+  else k_stencil_apply<ND, T, 3><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, S, u, f, dinv, omega, out);
+}
+template void stencil_apply<3, double>(const Launch&, int, int, const double*, const double*, const double*, const double*, double, double*, double*, hipStream_t);
+template void stencil_apply<1, double>(const Launch&, int, int, const double*, const double*, const double*, const double*, double, double*, double*, hipStream_t);
+template void stencil_apply<3, float>(const Launch&, int, int, const float*, const float*, const float*, const float*, float, float*, double*, hipStream_t);
+template void stencil_apply<1, float>(const Launch&, int, int, const float*, const float*, const float*, const float*, float, float*, double*, hipStream_t);
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_stencil_dinv(const LevelCell* __restrict__ cells,
+                                                      const int* __restrict__ blk_start, int ncells,
+                                                      long long Ntot, int nd, const T* __restrict__ S,
+                                                      T* __restrict__ dinv) {
+  LevelCell c; int ci, x, y, z; long long gid;
+  if (!node_of(cells, blk_start, ncells, blockIdx.x, c, ci, x, y, z, gid)) return;
+  for (int i = 0; i < nd; ++i) {
+    const T dv = S[(long long)(13 * nd * nd + i * nd + i) * Ntot + gid];
+    dinv[(long long)i * Ntot + gid] = dv != T(0) ? T(1) / dv : T(1);
+  }
+}
+
+template <typename T>
+void stencil_dinv(const Launch& L, int nd, const T* S, T* dinv, hipStream_t s) {
+  k_stencil_dinv<T><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, nd, S, dinv);
+}
+template void stencil_dinv<double>(const Launch&, int, const double*, double*, hipStream_t);
+template void stencil_dinv<float>(const Launch&, int, const float*, float*, hipStream_t);
+
+// blockIdx.y = neighbour offset d; one thread per coarse node.
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void k_stencil_level1(const LevelCell* __restrict__ ccells,
+                                                        const int* __restrict__ blk_start, int ncells,
+                                                        long long Ntot_c, const LevelCell* __restrict__ fcells,
+                                                        const double* __restrict__ G, const double* __restrict__ rho,
+                                                        T* __restrict__ S) {
+  constexpr int NE = 8 * ND;
+  LevelCell c; int ci, x, y, z; long long gid;
+  if (!node_of(ccells, blk_start, ncells, blockIdx.x, c, ci, x, y, z, gid)) return;
+  const LevelCell cf = fcells[ci];
+  const int d = blockIdx.y;
+  const int dd[3] = {d / 9 - 1, (d / 3) % 3 - 1, d % 3 - 1};
+  double acc[ND][ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+#pragma unroll
+    for (int j = 0; j < ND; ++j) acc[i][j] = 0.0;
+  for (int av = 0; av < 8; ++av) {
+    const int a[3] = {(av >> 2) & 1, (av >> 1) & 1, av & 1};
+    const int mx = dd[0] + 1 - a[0], my = dd[1] + 1 - a[1], mz = dd[2] + 1 - a[2];
+    if (mx < 0 || mx > 1 || my < 0 || my > 1 || mz < 0 || mz > 1) continue;
+    const int kN = hex_corner(1 - a[0], 1 - a[1], 1 - a[2]);
+    const int km = hex_corner(mx, my, mz);
+    const int Ex = wrap(x + a[0] - 1, c.nx), Ey = wrap(y + a[1] - 1, c.ny), Ez = wrap(z + a[2] - 1, c.nz);
+#pragma unroll
+    for (int ch = 0; ch < 8; ++ch) {
+      const long long fe = cf.off + ((long long)(2 * Ex + ((ch >> 2) & 1)) * cf.ny + (2 * Ey + ((ch >> 1) & 1))) * cf.nz +
+                           (2 * Ez + (ch & 1));
+      const double r = rho[fe];
+      const double* Gc = G + ((long long)ci * 8 + ch) * NE * NE;
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+#pragma unroll
+        for (int j = 0; j < ND; ++j) acc[i][j] = fma(r, Gc[(ND * kN + i) * NE + ND * km + j], acc[i][j]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+#pragma unroll
+    for (int j = 0; j < ND; ++j) S[(long long)(d * ND * ND + i * ND + j) * Ntot_c + gid] = (T)acc[i][j];
+}
+
+template <typename T>
+void stencil_level1(const Launch& Lc, const Launch& Lf, int nd, const double* G, const double* rho, T* S,
+                    hipStream_t s) {
+  dim3 grid(Lc.nblocks, 27);
+  if (nd == 3) k_stencil_level1<3, T><<<grid, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, G, rho, S);
+  else k_stencil_level1<1, T><<<grid, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, G, rho, S);
+}
+template void stencil_level1<double>(const Launch&, const Launch&, int, const double*, const double*, double*, hipStream_t);
+template void stencil_level1<float>(const Launch&, const Launch&, int, const double*, const double*, float*, hipStream_t);
+
+// Coarse block(N, D) = sum_{s,d} w(s) w(s+d-2D) Sf[d](2N+s), w(t) = prod_i w1(t_i).
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void k_stencil_galerkin(const LevelCell* __restrict__ ccells,
+                                                          const int* __restrict__ blk_start, int ncells,
+                                                          long long Ntot_c, const LevelCell* __restrict__ fcells,
+                                                          long long Ntot_f, const T* __restrict__ Sf,
+                                                          T* __restrict__ Sc) {
+  LevelCell c; int ci, x, y, z; long long gid;
+  if (!node_of(ccells, blk_start, ncells, blockIdx.x, c, ci, x, y, z, gid)) return;
+  const LevelCell cf = fcells[ci];
+  const int D = blockIdx.y;
+  const int Dx = D / 9 - 1, Dy = (D / 3) % 3 - 1, Dz = D % 3 - 1;
+  double acc[ND][ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+#pragma unroll
+    for (int j = 0; j < ND; ++j) acc[i][j] = 0.0;
+  for (int sx = -1; sx <= 1; ++sx)
+    for (int dx = -1; dx <= 1; ++dx) {
+      const double wx = w1(sx) * w1(sx + dx - 2 * Dx);
+      if (wx == 0.0) continue;
+      const int fx = wrap(2 * x + sx, cf.nx);
+      for (int sy = -1; sy <= 1; ++sy)
+        for (int dy = -1; dy <= 1; ++dy) {
+          const double wy = wx * w1(sy) * w1(sy + dy - 2 * Dy);
+          if (wy == 0.0) continue;
+          const int fy = wrap(2 * y + sy, cf.ny);
+          for (int sz = -1; sz <= 1; ++sz)
+            for (int dz = -1; dz <= 1; ++dz) {
+              const double w = wy * w1(sz) * w1(sz + dz - 2 * Dz);
+              if (w == 0.0) continue;
+              const int fz = wrap(2 * z + sz, cf.nz);
+              const long long fn = cf.off + ((long long)fx * cf.ny + fy) * cf.nz + fz;
+              const int d = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+#pragma unroll
+              for (int i = 0; i < ND; ++i)
+#pragma unroll
+                for (int j = 0; j < ND; ++j)
+                  acc[i][j] = fma(w, (double)Sf[(long long)(d * ND * ND + i * ND + j) * Ntot_f + fn], acc[i][j]);
+            }
+        }
+    }
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+#pragma unroll
+    for (int j = 0; j < ND; ++j) Sc[(long long)(D * ND * ND + i * ND + j) * Ntot_c + gid] = (T)acc[i][j];
+}
+
+template <typename T>
+void stencil_galerkin(const Launch& Lc, const Launch& Lf, int nd, const T* Sf, T* Sc, hipStream_t s) {
+  dim3 grid(Lc.nblocks, 27);
+  if (nd == 3) k_stencil_galerkin<3, T><<<grid, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, Lf.Ntot, Sf, Sc);
+  else k_stencil_galerkin<1, T><<<grid, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, Lf.Ntot, Sf, Sc);
+}
+template void stencil_galerkin<double>(const Launch&, const Launch&, int, const double*, double*, hipStream_t);
+template void stencil_galerkin<float>(const Launch&, const Launch&, int, const float*, float*, hipStream_t);
+
+// b_c(N) = sum_{s in {-1,0,1}^3} w(s) r_f(2N+s)
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void k_restrict(const LevelCell* __restrict__ ccells,
+                                                  const int* __restrict__ blk_start, int ncells, long long Ntot_c,
+                                                  const LevelCell* __restrict__ fcells, long long Ntot_f, int ncase,
+                                                  const T* __restrict__ rf, T* __restrict__ bc) {
+  LevelCell c; int ci, x, y, z; long long gid;
+  if (!node_of(ccells, blk_start, ncells, blockIdx.x, c, ci, x, y, z, gid)) return;
+  const LevelCell cf = fcells[ci];
+  T acc[MAPLE_MAX_CASES][ND];
+#pragma unroll
+  for (int j = 0; j < MAPLE_MAX_CASES; ++j)
+#pragma unroll
+    for (int i = 0; i < ND; ++i) acc[j][i] = T(0);
+  for (int sx = -1; sx <= 1; ++sx) {
+    const int fx = wrap(2 * x + sx, cf.nx);
+    for (int sy = -1; sy <= 1; ++sy) {
+      const int fy = wrap(2 * y + sy, cf.ny);
+#pragma unroll
+      for (int sz = -1; sz <= 1; ++sz) {
+        const int fz = wrap(2 * z + sz, cf.nz);
+        const T w = (T)(w1(sx) * w1(sy) * w1(sz));
+        const long long fn = cf.off + ((long long)fx * cf.ny + fy) * cf.nz + fz;
+#pragma unroll
+        for (int j = 0; j < MAPLE_MAX_CASES; ++j)
+          if (j < ncase)
+#pragma unroll
+            for (int i = 0; i < ND; ++i) acc[j][i] = fma(w, rf[(long long)(j * ND + i) * Ntot_f + fn], acc[j][i]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < MAPLE_MAX_CASES; ++j)
+    if (j < ncase)
+#pragma unroll
+      for (int i = 0; i < ND; ++i) bc[(long long)(j * ND + i) * Ntot_c + gid] = acc[j][i];
+}
+
+template <typename T>
+void restrict_pt(const Launch& Lc, const Launch& Lf, int nd, int ncase, const T* rf, T* bc, hipStream_t s) {
+  if (nd == 3) k_restrict<3, T><<<Lc.nblocks, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, Lf.Ntot, ncase, rf, bc);
+  else k_restrict<1, T><<<Lc.nblocks, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, Lf.Ntot, ncase, rf, bc);
+}
+template void restrict_pt<double>(const Launch&, const Launch&, int, int, const double*, double*, hipStream_t);
+template void restrict_pt<float>(const Launch&, const Launch&, int, int, const float*, float*, hipStream_t);
+
+// u_f(n) += sum over the <= 8 coarse nodes that interpolate to n
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void k_prolong_add(const LevelCell* __restrict__ fcells,
+                                                     const int* __restrict__ blk_start, int ncells, long long Ntot_f,
+                                                     const LevelCell* __restrict__ ccells, long long Ntot_c, int ncase,
+                                                     const T* __restrict__ ec, T* __restrict__ uf) {
+  LevelCell c; int ci, x, y, z; long long gid;
+  if (!node_of(fcells, blk_start, ncells, blockIdx.x, c, ci, x, y, z, gid)) return;
+  const LevelCell cc = ccells[ci];
+  const int bx = x >> 1, by = y >> 1, bz = z >> 1;
+  const int rx = x & 1, ry = y & 1, rz = z & 1;
+  T acc[MAPLE_MAX_CASES][ND];
+#pragma unroll
+  for (int j = 0; j < MAPLE_MAX_CASES; ++j)
+#pragma unroll
+    for (int i = 0; i < ND; ++i) acc[j][i] = T(0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int ox = (k >> 2) & 1, oy = (k >> 1) & 1, oz = k & 1;
+    if ((ox && !rx) || (oy && !ry) || (oz && !rz)) continue;
+    const T w = (T)((rx ? 0.5 : 1.0) * (ry ? 0.5 : 1.0) * (rz ? 0.5 : 1.0));
+    const long long cn = cc.off + ((long long)wrap(bx + ox, cc.nx) * cc.ny + wrap(by + oy, cc.ny)) * cc.nz +
+                         wrap(bz + oz, cc.nz);
+#pragma unroll
+    for (int j = 0; j < MAPLE_MAX_CASES; ++j)
+      if (j < ncase)
+#pragma unroll
+        for (int i = 0; i < ND; ++i) acc[j][i] = fma(w, ec[(long long)(j * ND + i) * Ntot_c + cn], acc[j][i]);
+  }
+#pragma unroll
+  for (int j = 0; j < MAPLE_MAX_CASES; ++j)
+    if (j < ncase)
+#pragma unroll
+      for (int i = 0; i < ND; ++i) uf[(long long)(j * ND + i) * Ntot_f + gid] += acc[j][i];
+}
+
+template <typename T>
+void prolong_add(const Launch& Lf, const Launch& Lc, int nd, int ncase, const T* ec, T* uf, hipStream_t s) {
+  if (nd == 3) k_prolong_add<3, T><<<Lf.nblocks, 256, 0, s>>>(Lf.cells, Lf.blk_start, Lf.ncells, Lf.Ntot, Lc.cells, Lc.Ntot, ncase, ec, uf);
+  else k_prolong_add<1, T><<<Lf.nblocks, 256, 0, s>>>(Lf.cells, Lf.blk_start, Lf.ncells, Lf.Ntot, Lc.cells, Lc.Ntot, ncase, ec, uf);
+}
+template void prolong_add<double>(const Launch&, const Launch&, int, int, const double*, double*, hipStream_t);
+template void prolong_add<float>(const Launch&, const Launch&, int, int, const float*, float*, hipStream_t);

@@ -1,0 +1,276 @@
+// Coarsest multigrid level: exact solve with a dense SPD inverse.
+//
+// The periodic operator is singular (3 rigid translations for elasticity, the
+// constant for heat).  The coarsest matrix is made SPD with the translation
+// constraint, A' = A + gamma Z Z^T (Z the normalised translations, gamma the
+// mean diagonal); for a right-hand side orthogonal to Z the solution of
+// A' x = b is the minimum-norm solution of A x = b.  A'^-1 is formed once per
+// density field by an in-place blocked Gauss-Jordan sweep (64 x 64 tiles, no
+// pivoting, valid for SPD) and each V-cycle applies it as a dense mat-vec over
+// all load cases, which keeps the coarsest solve a bandwidth-bound GEMV
+// instead of two latency-bound triangular solves.
+#include "kernels.h"
+
+#define TB 64
+
+// gamma[cell] = trace(A)/n from the centre blocks of the coarsest stencil.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_gamma(const LevelCell* __restrict__ cells,
+                                                     const DenseCell* __restrict__ dcells, long long Ntot,
+                                                     int nd, const T* __restrict__ S, double* __restrict__ gamma) {
+  const int ci = blockIdx.x;
+  const LevelCell c = cells[ci];
+  const long long nn = (long long)c.nx * c.ny * c.nz;
+  double v[1] = {0.0};
+  for (long long n = threadIdx.x; n < nn; n += blockDim.x)
+    for (int i = 0; i < nd; ++i) v[0] += (double)S[(long long)(13 * nd * nd + i * nd + i) * Ntot + c.off + n];
+  __shared__ double tot;
+  block_sum<1>(v, &tot);
+  if (threadIdx.x == 0) gamma[ci] = tot / (double)dcells[ci].n;
+}
+
+// A[(n,i),(m,j)] = sum_{d: n+d == m} S[d](i,j)(n) + gamma/N delta_ij
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_assemble(const LevelCell* __restrict__ cells,
+                                                        const DenseCell* __restrict__ dcells, long long Ntot,
+                                                        int nd, const T* __restrict__ S,
+                                                        const double* __restrict__ gamma, double* __restrict__ A) {
+  const int ci = blockIdx.y;
+  const LevelCell c = cells[ci];
+  const DenseCell dc = dcells[ci];
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long n2 = (long long)dc.n * dc.n;
+  if (e >= n2) return;
+  const int row = (int)(e / dc.n), col = (int)(e - (long long)row * dc.n);
+  const int nr = row / nd, ir = row - nr * nd;
+  const int mc = col / nd, jc = col - mc * nd;
+  const int nz = c.nz, ny = c.ny;
+  const int zr = nr % nz, yr = (nr / nz) % ny, xr = nr / (nz * ny);
+  const int zc = mc % nz, yc = (mc / nz) % ny, xc = mc / (nz * ny);
+  double acc = 0.0;
+  for (int dx = -1; dx <= 1; ++dx) {
+    if (wrap(xr + dx, c.nx) != xc) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      if (wrap(yr + dy, c.ny) != yc) continue;
+      for (int dz = -1; dz <= 1; ++dz) {
+        if (wrap(zr + dz, c.nz) != zc) continue;
+        const int d = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+        acc += (double)S[(long long)(d * nd * nd + ir * nd + jc) * Ntot + c.off + nr];
+      }
+    }
+  }
+  const long long nn = (long long)c.nx * c.ny * c.nz;
+  if (ir == jc) acc += gamma[ci] / (double)nn;
+  A[dc.aoff + e] = acc;
+}
+
+template <typename T>
+void dense_assemble_g(const Launch& Lc, const DenseCell* dcells, int maxn, int nd, const T* S, double* A,
+                      double* gamma, hipStream_t s) {
+  k_dense_gamma<T><<<Lc.ncells, 256, 0, s>>>(Lc.cells, dcells, Lc.Ntot, nd, S, gamma);
+  dim3 grid((unsigned)(((long long)maxn * maxn + 255) / 256), Lc.ncells);
+  k_dense_assemble<T><<<grid, 256, 0, s>>>(Lc.cells, dcells, Lc.Ntot, nd, S, gamma, A);
+}
+template void dense_assemble_g<double>(const Launch&, const DenseCell*, int, int, const double*, double*, double*, hipStream_t);
+template void dense_assemble_g<float>(const Launch&, const DenseCell*, int, int, const float*, double*, double*, hipStream_t);
+
+// ---- blocked Gauss-Jordan ------------------------------------------------------
+
+__device__ __forceinline__ int tile_rows(const DenseCell& dc, int t) { return min(TB, dc.n - t * TB); }
+
+// load tile (ti,tj) of cell matrix into LDS (zero padded)
+__device__ __forceinline__ void load_tile(const DenseCell& dc, const double* __restrict__ A, int ti, int tj,
+                                          double (*L)[TB]) {
+  const int nr = tile_rows(dc, ti), nc = tile_rows(dc, tj);
+  for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
+    const int r = e / TB, cc = e - r * TB;
+    L[r][cc] = (r < nr && cc < nc) ? A[dc.aoff + (long long)(ti * TB + r) * dc.n + tj * TB + cc] : 0.0;
+  }
+}
+
+// K1: invert the pivot tile in LDS (scalar Gauss-Jordan), write it to work[cell].
+__global__ __launch_bounds__(256) void k_gj_diag(const DenseCell* __restrict__ dcells, int k,
+                                                 const double* __restrict__ A, double* __restrict__ work) {
+  const int ci = blockIdx.x;
+  const DenseCell dc = dcells[ci];
+  if (k >= dc.nt) return;
+  __shared__ double M[TB][TB];
+  load_tile(dc, A, k, k, M);
+  const int nb = tile_rows(dc, k);
+  // pad the identity below the matrix so the padded pivots are 1
+  for (int r = threadIdx.x; r < TB; r += blockDim.x)
+    if (r >= nb) M[r][r] = 1.0;
+  __syncthreads();
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;   // 16 x 16 threads, 4 x 4 entries each
+  for (int p = 0; p < TB; ++p) {
+    const double pv = 1.0 / M[p][p];
+    double nv[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int i = tr * 4 + a, j = tc * 4 + b;
+        const double mij = M[i][j], mip = M[i][p], mpj = M[p][j];
+        if (i == p && j == p) nv[a][b] = pv;
+        else if (i == p) nv[a][b] = mpj * pv;
+        else if (j == p) nv[a][b] = -mip * pv;
+        else nv[a][b] = mij - mip * mpj * pv;
+      }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) M[tr * 4 + a][tc * 4 + b] = nv[a][b];
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) work[(long long)ci * TB * TB + e] = M[e / TB][e % TB];
+}
+
+// C(64x64) = X * Y with X, Y in LDS; each thread 4x4 outputs.
+__device__ __forceinline__ void tile_mm(double (*X)[TB], double (*Y)[TB], double (&acc)[4][4]) {
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  for (int kk = 0; kk < TB; ++kk) {
+    double xa[4], yb[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) xa[a] = X[tr * 4 + a][kk];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) yb[b] = Y[kk][tc * 4 + b];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = fma(xa[a], yb[b], acc[a][b]);
+  }
+}
+
+__device__ __forceinline__ void store_tile(const DenseCell& dc, double* __restrict__ A, int ti, int tj,
+                                           const double (&v)[4][4], double scale, bool accumulate) {
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  const int nr = tile_rows(dc, ti), nc = tile_rows(dc, tj);
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int r = tr * 4 + a, cc = tc * 4 + b;
+      if (r < nr && cc < nc) {
+        double* p = A + dc.aoff + (long long)(ti * TB + r) * dc.n + tj * TB + cc;
+        *p = accumulate ? *p + scale * v[a][b] : scale * v[a][b];
+      }
+    }
+}
+
+__device__ __forceinline__ void load_work(const double* __restrict__ work, int ci, double (*L)[TB]) {
+  for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) L[e / TB][e % TB] = work[(long long)ci * TB * TB + e];
+}
+
+// K2: row panel A_kj <- P A_kj (j != k)
+__global__ __launch_bounds__(256) void k_gj_row(const DenseCell* __restrict__ dcells, int k, double* __restrict__ A,
+                                                const double* __restrict__ work) {
+  const int ci = blockIdx.y, j = blockIdx.x;
+  const DenseCell dc = dcells[ci];
+  if (k >= dc.nt || j >= dc.nt || j == k) return;
+  __shared__ double P[TB][TB];
+  __shared__ double B[TB][TB];
+  load_work(work, ci, P);
+  load_tile(dc, A, k, j, B);
+  __syncthreads();
+  double acc[4][4];
+  tile_mm(P, B, acc);
+  store_tile(dc, A, k, j, acc, 1.0, false);
+}
+
+// K3: A_ij -= A_ik A_kj (i != k, j != k), A_kj already updated
+__global__ __launch_bounds__(256) void k_gj_update(const DenseCell* __restrict__ dcells, int k,
+                                                   double* __restrict__ A) {
+  const int ci = blockIdx.z, i = blockIdx.y, j = blockIdx.x;
+  const DenseCell dc = dcells[ci];
+  if (k >= dc.nt || i >= dc.nt || j >= dc.nt || i == k || j == k) return;
+  __shared__ double X[TB][TB];
+  __shared__ double Y[TB][TB];
+  load_tile(dc, A, i, k, X);
+  load_tile(dc, A, k, j, Y);
+  __syncthreads();
+  double acc[4][4];
+  tile_mm(X, Y, acc);
+  store_tile(dc, A, i, j, acc, -1.0, true);
+}
+
+// K4: A_ik <- -A_ik P (i != k), A_kk <- P
+__global__ __launch_bounds__(256) void k_gj_col(const DenseCell* __restrict__ dcells, int k, double* __restrict__ A,
+                                                const double* __restrict__ work) {
+  const int ci = blockIdx.y, i = blockIdx.x;
+  const DenseCell dc = dcells[ci];
+  if (k >= dc.nt || i >= dc.nt) return;
+  __shared__ double P[TB][TB];
+  __shared__ double X[TB][TB];
+  load_work(work, ci, P);
+  if (i == k) {
+    __syncthreads();
+    const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+    double v[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) v[a][b] = P[tr * 4 + a][tc * 4 + b];
+    store_tile(dc, A, k, k, v, 1.0, false);
+    return;
+  }
+  load_tile(dc, A, i, k, X);
+  __syncthreads();
+  double acc[4][4];
+  tile_mm(X, P, acc);
+  store_tile(dc, A, i, k, acc, -1.0, false);
+}
+
+void dense_inverse(const DenseCell* dcells, int ncells, int max_nt, double* A, double* work, hipStream_t s) {
+  for (int k = 0; k < max_nt; ++k) {
+    k_gj_diag<<<ncells, 256, 0, s>>>(dcells, k, A, work);
+    k_gj_row<<<dim3(max_nt, ncells), 256, 0, s>>>(dcells, k, A, work);
+    k_gj_update<<<dim3(max_nt, max_nt, ncells), 256, 0, s>>>(dcells, k, A);
+    k_gj_col<<<dim3(max_nt, ncells), 256, 0, s>>>(dcells, k, A, work);
+  }
+}
+
+// x_j = Ainv b_j for every case; one wavefront per matrix row.
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void k_dense_apply(const DenseCell* __restrict__ dcells, long long Ntot,
+                                                     int ncase, const double* __restrict__ Ainv,
+                                                     const T* __restrict__ b, T* __restrict__ x) {
+  const int ci = blockIdx.y;
+  const DenseCell dc = dcells[ci];
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= dc.n) return;
+  double acc[MAPLE_MAX_CASES];
+#pragma unroll
+  for (int j = 0; j < MAPLE_MAX_CASES; ++j) acc[j] = 0.0;
+  const double* Ar = Ainv + dc.aoff + (long long)row * dc.n;
+  for (int col = lane; col < dc.n; col += 64) {
+    const double a = Ar[col];
+    const int m = col / ND, i = col - m * ND;
+#pragma unroll
+    for (int j = 0; j < MAPLE_MAX_CASES; ++j)
+      if (j < ncase) acc[j] = fma(a, (double)b[(long long)(j * ND + i) * Ntot + dc.voff + m], acc[j]);
+  }
+  const int m = row / ND, i = row - m * ND;
+#pragma unroll
+  for (int j = 0; j < MAPLE_MAX_CASES; ++j) {
+    if (j < ncase) {
+      const double v = wave_sum(acc[j]);
+      if (lane == 0) x[(long long)(j * ND + i) * Ntot + dc.voff + m] = (T)v;
+    }
+  }
+}
+
+template <typename T>
+void dense_apply(const Launch& Lc, const DenseCell* dcells, int max_n, int nd, int ncase, const double* Ainv,
+                 const T* b, T* x, hipStream_t s) {
+  dim3 grid((max_n + 3) / 4, Lc.ncells);
+  if (nd == 3) k_dense_apply<3, T><<<grid, 256, 0, s>>>(dcells, Lc.Ntot, ncase, Ainv, b, x);
+  else k_dense_apply<1, T><<<grid, 256, 0, s>>>(dcells, Lc.Ntot, ncase, Ainv, b, x);
+}
+template void dense_apply<double>(const Launch&, const DenseCell*, int, int, int, const double*, const double*, double*, hipStream_t);
+template void dense_apply<float>(const Launch&, const DenseCell*, int, int, int, const double*, const float*, float*, hipStream_t);

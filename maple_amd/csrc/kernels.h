@@ -1,0 +1,123 @@
+// Kernel launchers of the cell-solver library (device code lives in *.hip).
+#pragma once
+
+#include "common.h"
+
+// A launch over every cell of a batch: blk_start (device, ncells+1 entries)
+// maps a block to its cell; the blocks of one cell are contiguous.
+struct Launch {
+  const LevelCell* cells = nullptr;
+  const int* blk_start = nullptr;
+  int ncells = 0;
+  int nblocks = 0;
+  long long Ntot = 0;
+};
+
+// ---- level 0 (fine.hip) ---------------------------------------------------
+int fine_waves(const LevelCell& c);  // wavefronts per cell of k_fine_apply (4 per block)
+
+// mode 0: out = K u; 1: out = K u, partial[blk*6+j] = (u_j, K u_j) over the block;
+// 2: out = f - K u; 3: out = u + omega Dinv (f - K u)
+template <int ND, typename T>
+void fine_apply(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u,
+                const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s);
+void fine_rhs(const Launch& L, int nd, int ncase, const double* Fe, const double* rho, double* F, hipStream_t s);
+template <typename T>
+void fine_dinv(const Launch& L, int nd, const double* Ke, const double* rho, T* dinv, hipStream_t s);
+
+// ---- coarse levels (coarse.hip) --------------------------------------------
+// Stencil S of a level: S[(d*ND*ND + i*ND + j)*Ntot + n] is the (i,j) entry of the
+// block coupling node n to node n+off(d), d = (dx+1)*9 + (dy+1)*3 + (dz+1).
+template <int ND, typename T>
+void stencil_apply(const Launch& L, int mode, int ncase, const T* S, const T* u, const T* f,
+                   const T* dinv, T omega, T* out, double* partial, hipStream_t s);
+template <typename T>
+void stencil_dinv(const Launch& L, int nd, const T* S, T* dinv, hipStream_t s);
+// level-1 stencil straight from the voxel densities: block(N, d) =
+// sum over coarse elements E containing N and N+d of sum_c rho_child(E,c) G_c[..]
+template <typename T>
+void stencil_level1(const Launch& Lc, const Launch& Lf, int nd, const double* G, const double* rho, T* S, hipStream_t s);
+// Galerkin P^T A P on stencils (fine stencil Sf -> coarse stencil Sc)
+template <typename T>
+void stencil_galerkin(const Launch& Lc, const Launch& Lf, int nd, const T* Sf, T* Sc, hipStream_t s);
+// b_c = P^T r_f (periodic trilinear); out_f += P e_c
+template <typename T>
+void restrict_pt(const Launch& Lc, const Launch& Lf, int nd, int ncase, const T* rf, T* bc, hipStream_t s);
+template <typename T>
+void prolong_add(const Launch& Lf, const Launch& Lc, int nd, int ncase, const T* ec, T* uf, hipStream_t s);
+
+// ---- dense coarsest level (dense.hip) --------------------------------------
+struct DenseCell {
+  int n;            // matrix order (ND * nodes)
+  int nt;           // tiles per side (n / 64 rounded up)
+  long long aoff;   // offset of the n x n row-major matrix in the dense buffer
+  long long voff;   // first node of the cell in the coarsest-level arrays
+};
+// gamma: ncells doubles of scratch (the constraint weight of each cell)
+template <typename T>
+void dense_assemble_g(const Launch& Lc, const DenseCell* dcells, int maxn, int nd, const T* S, double* A,
+                      double* gamma, hipStream_t s);
+// In-place blocked Gauss-Jordan inverse of SPD matrices, one step per pivot tile.
+void dense_inverse(const DenseCell* dcells, int ncells, int max_nt, double* A, double* work, hipStream_t s);
+template <typename T>
+void dense_apply(const Launch& Lc, const DenseCell* dcells, int max_n, int nd, int ncase, const double* Ainv,
+                 const T* b, T* x, hipStream_t s);
+
+// ---- vectors and PCG scalars (vec.hip) --------------------------------------
+struct CaseScalars {   // one per (cell, case)
+  double ff;           // ||f||^2
+  double tgt2;         // squared residual target
+  double rz, rz_new, pq, rr;
+  double alpha, beta;
+  int done, iters;
+  int pad_[2];
+};
+// sum partial[blk*6 + j] over each cell's blocks into dst (a member of CaseScalars)
+void reduce_partials(const int* blk_start, int ncells, int ncase, const double* partial,
+                     CaseScalars* sc, int which, hipStream_t s);
+enum { SC_FF = 0, SC_RZ_NEW, SC_PQ, SC_RR };
+// elementwise (cell-aligned 256-node blocks):
+template <typename T>
+void vec_scale_dinv(const Launch& L, int nd, int ncase, T omega, const T* dinv, const T* b, T* x, hipStream_t s);
+// partial (a_j, b_j)
+template <typename Ta, typename Tb>
+void vec_dot(const Launch& L, int nd, int ncase, const Ta* a, const Tb* b, double* partial, hipStream_t s);
+// u += alpha p; r -= alpha q; partial (r, r)
+void pcg_update_ur(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* p, const double* q,
+                   double* u, double* r, double* partial, hipStream_t s);
+// p = z + beta p
+void pcg_update_p(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* z, double* p, hipStream_t s);
+// per (cell, case) scalar steps
+void pcg_init_scalars(int ncells, int ncase, double rtol, const double* atol2, CaseScalars* sc, int* done_flags,
+                      hipStream_t s);
+void pcg_alpha(int ncells, int ncase, CaseScalars* sc, hipStream_t s);
+void pcg_check(int ncells, int ncase, CaseScalars* sc, int* done_flags, hipStream_t s);
+void pcg_beta(int ncells, int ncase, CaseScalars* sc, hipStream_t s);
+template <typename Td, typename Ts>
+void vec_copy(long long n, const Ts* src, Td* dst, hipStream_t s);
+template <typename T>
+void vec_zero(long long n, T* dst, hipStream_t s);
+
+// ---- homogenization, sensitivities, filter, OC (homog.hip) -----------------
+// S_e packed upper triangle (nu = nc(nc+1)/2 values per element) and
+// per-block partials of rho_e S_e
+void element_energy(const Launch& L, int nd, int ncase, const double* Ke, const double* X0, const double* rho,
+                    const double* u, double* Se, double* partial, hipStream_t s);
+void reduce_energy(const int* blk_start, int ncells, int nu, const double* partial, int nc, double* CH, hipStream_t s);
+void sensitivity(const Launch& L, int nu3, const double* Se3, const double* G6, int nu1, const double* Se1,
+                 const double* G3, double* dJdrho, hipStream_t s);
+void box_filter(const Launch& L, int k, const double* in, double* out, hipStream_t s);
+struct OCState {
+  double l1, l2, last_lam;
+  double volfrac, move, xmin;
+  double lam[64];      // the 63 midpoints of this pass (heap order, index 1..63)
+  double c[64];        // 1/sqrt(lam)
+};
+// a = x sqrt(max(dJ, 1e-12)); 256-element blocks
+void oc_prepare(const Launch& L, const double* x, const double* dJ, double* a, hipStream_t s);
+void oc_start(int ncells, OCState* st, hipStream_t s);
+// 1024-element blocks; partial[blk*64 + t]
+void oc_pass(const Launch& L, const OCState* st, const double* x, const double* a, double* partial, hipStream_t s);
+void oc_walk(const int* blk_start, const LevelCell* cells, int ncells, int depth, const double* partial,
+             OCState* st, hipStream_t s);
+void oc_final(const Launch& L, const OCState* st, const double* x, const double* a, double* xnew, hipStream_t s);

@@ -1,0 +1,219 @@
+// Vector and scalar kernels of the multi-right-hand-side preconditioned CG.
+//
+// All load cases of all cells advance together; every (cell, case) pair keeps
+// its own CG scalars on the device (CaseScalars), so no value goes to the host
+// inside an iteration.  A converged pair gets alpha = beta = 0 and stays fixed
+// while the others finish.  Dot products are reduced deterministically:
+// per-block partials in a fixed order, then one wavefront per (cell, case).
+#include "kernels.h"
+
+__global__ __launch_bounds__(64) void k_reduce_partials(const int* __restrict__ blk_start, int ncase,
+                                                        const double* __restrict__ partial,
+                                                        CaseScalars* __restrict__ sc, int which) {
+  const int ci = blockIdx.x, j = blockIdx.y;
+  const int b0 = blk_start[ci], b1 = blk_start[ci + 1];
+  double v = 0.0;
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += 64) v += partial[(long long)b * MAPLE_MAX_CASES + j];
+  v = wave_sum(v);
+  if (threadIdx.x == 0) {
+    CaseScalars& s = sc[ci * ncase + j];
+    if (which == SC_FF) s.ff = v;
+    else if (which == SC_RZ_NEW) s.rz_new = v;
+    else if (which == SC_PQ) s.pq = v;
+    else s.rr = v;
+  }
+}
+
+void reduce_partials(const int* blk_start, int ncells, int ncase, const double* partial, CaseScalars* sc,
+                     int which, hipStream_t s) {
+  k_reduce_partials<<<dim3(ncells, ncase), 64, 0, s>>>(blk_start, ncase, partial, sc, which);
+}
+
+__device__ __forceinline__ bool vnode(const Launch& L, int& ci, long long& gid) {
+  ci = find_cell(L.blk_start, L.ncells, blockIdx.x);
+  const LevelCell c = L.cells[ci];
+  const long long n = (long long)(blockIdx.x - L.blk_start[ci]) * 256 + threadIdx.x;
+  if (n >= (long long)c.nx * c.ny * c.nz) return false;
+  gid = c.off + n;
+  return true;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_scale_dinv(Launch L, int nd, int ncase, T omega,
+                                                    const T* __restrict__ dinv, const T* __restrict__ b,
+                                                    T* __restrict__ x) {
+  int ci; long long gid;
+  if (!vnode(L, ci, gid)) return;
+  for (int i = 0; i < nd; ++i) {
+    const T w = omega * dinv[(long long)i * L.Ntot + gid];
+    for (int j = 0; j < ncase; ++j) {
+      const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
+      x[idx] = w * b[idx];
+    }
+  }
+}
+
+template <typename T>
+void vec_scale_dinv(const Launch& L, int nd, int ncase, T omega, const T* dinv, const T* b, T* x, hipStream_t s) {
+  k_scale_dinv<T><<<L.nblocks, 256, 0, s>>>(L, nd, ncase, omega, dinv, b, x);
+}
+template void vec_scale_dinv<double>(const Launch&, int, int, double, const double*, const double*, double*, hipStream_t);
+template void vec_scale_dinv<float>(const Launch&, int, int, float, const float*, const float*, float*, hipStream_t);
+
+template <typename Ta, typename Tb>
+__global__ __launch_bounds__(256) void k_dot(Launch L, int nd, int ncase, const Ta* __restrict__ a,
+                                             const Tb* __restrict__ b, double* __restrict__ partial) {
+  int ci; long long gid = 0;
+  const bool ok = vnode(L, ci, gid);
+  for (int j = 0; j < ncase; ++j) {
+    double v[1] = {0.0};
+    if (ok)
+      for (int i = 0; i < nd; ++i) {
+        const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
+        v[0] += (double)a[idx] * (double)b[idx];
+      }
+    block_sum<1>(v, partial + (long long)blockIdx.x * MAPLE_MAX_CASES + j);
+  }
+}
+
+template <typename Ta, typename Tb>
+void vec_dot(const Launch& L, int nd, int ncase, const Ta* a, const Tb* b, double* partial, hipStream_t s) {
+  k_dot<Ta, Tb><<<L.nblocks, 256, 0, s>>>(L, nd, ncase, a, b, partial);
+}
+template void vec_dot<double, double>(const Launch&, int, int, const double*, const double*, double*, hipStream_t);
+template void vec_dot<double, float>(const Launch&, int, int, const double*, const float*, double*, hipStream_t);
+template void vec_dot<float, float>(const Launch&, int, int, const float*, const float*, double*, hipStream_t);
+
+__global__ __launch_bounds__(256) void k_update_ur(Launch L, int nd, int ncase, const CaseScalars* __restrict__ sc,
+                                                   const double* __restrict__ p, const double* __restrict__ q,
+                                                   double* __restrict__ u, double* __restrict__ r,
+                                                   double* __restrict__ partial) {
+  int ci; long long gid = 0;
+  const bool ok = vnode(L, ci, gid);
+  for (int j = 0; j < ncase; ++j) {
+    double v[1] = {0.0};
+    if (ok) {
+      const double al = sc[ci * ncase + j].alpha;
+      for (int i = 0; i < nd; ++i) {
+        const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
+        u[idx] = fma(al, p[idx], u[idx]);
+        const double rn = fma(-al, q[idx], r[idx]);
+        r[idx] = rn;
+        v[0] += rn * rn;
+      }
+    }
+    block_sum<1>(v, partial + (long long)blockIdx.x * MAPLE_MAX_CASES + j);
+  }
+}
+
+void pcg_update_ur(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* p, const double* q,
+                   double* u, double* r, double* partial, hipStream_t s) {
+  k_update_ur<<<L.nblocks, 256, 0, s>>>(L, nd, ncase, sc, p, q, u, r, partial);
+}
+
+__global__ __launch_bounds__(256) void k_update_p(Launch L, int nd, int ncase, const CaseScalars* __restrict__ sc,
+                                                  const double* __restrict__ z, double* __restrict__ p) {
+  int ci; long long gid;
+  if (!vnode(L, ci, gid)) return;
+  for (int j = 0; j < ncase; ++j) {
+    const double be = sc[ci * ncase + j].beta;
+    for (int i = 0; i < nd; ++i) {
+      const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
+      p[idx] = fma(be, p[idx], z[idx]);
+    }
+  }
+}
+
+void pcg_update_p(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* z, double* p,
+                  hipStream_t s) {
+  k_update_p<<<L.nblocks, 256, 0, s>>>(L, nd, ncase, sc, z, p);
+}
+
+// ff and rr must already hold ||f||^2 and ||r0||^2.
+__global__ void k_pcg_init(int n, int ncase, double rtol, const double* __restrict__ atol2,
+                           CaseScalars* __restrict__ sc, int* __restrict__ done_flags) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  CaseScalars& s = sc[t];
+  s.tgt2 = fmax(rtol * rtol * s.ff, atol2[t / ncase]);
+  s.done = s.rr <= s.tgt2 ? 1 : 0;
+  s.iters = 0;
+  s.alpha = s.beta = 0.0;
+  s.rz = s.rz_new;
+  done_flags[t] = s.done;
+}
+
+void pcg_init_scalars(int ncells, int ncase, double rtol, const double* atol2, CaseScalars* sc, int* done_flags,
+                      hipStream_t s) {
+  const int n = ncells * ncase;
+  k_pcg_init<<<(n + 63) / 64, 64, 0, s>>>(n, ncase, rtol, atol2, sc, done_flags);
+}
+
+__global__ void k_pcg_alpha(int n, CaseScalars* __restrict__ sc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  CaseScalars& s = sc[t];
+  s.alpha = (!s.done && s.pq > 0.0) ? s.rz / s.pq : 0.0;
+}
+
+void pcg_alpha(int ncells, int ncase, CaseScalars* sc, hipStream_t s) {
+  const int n = ncells * ncase;
+  k_pcg_alpha<<<(n + 63) / 64, 64, 0, s>>>(n, sc);
+}
+
+__global__ void k_pcg_check(int n, CaseScalars* __restrict__ sc, int* __restrict__ done_flags) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  CaseScalars& s = sc[t];
+  if (!s.done) {
+    s.iters += 1;
+    if (s.rr <= s.tgt2 || !(s.rr == s.rr)) s.done = 1;
+  }
+  done_flags[t] = s.done;
+}
+
+void pcg_check(int ncells, int ncase, CaseScalars* sc, int* done_flags, hipStream_t s) {
+  const int n = ncells * ncase;
+  k_pcg_check<<<(n + 63) / 64, 64, 0, s>>>(n, sc, done_flags);
+}
+
+__global__ void k_pcg_beta(int n, CaseScalars* __restrict__ sc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  CaseScalars& s = sc[t];
+  s.beta = (!s.done && s.rz != 0.0) ? s.rz_new / s.rz : 0.0;
+  s.rz = s.rz_new;
+}
+
+void pcg_beta(int ncells, int ncase, CaseScalars* sc, hipStream_t s) {
+  const int n = ncells * ncase;
+  k_pcg_beta<<<(n + 63) / 64, 64, 0, s>>>(n, sc);
+}
+
+template <typename Td, typename Ts>
+__global__ void k_copy(long long n, const Ts* __restrict__ src, Td* __restrict__ dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (Td)src[i];
+}
+
+template <typename Td, typename Ts>
+void vec_copy(long long n, const Ts* src, Td* dst, hipStream_t s) {
+  if (n > 0) k_copy<Td, Ts><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, src, dst);
+}
+template void vec_copy<double, double>(long long, const double*, double*, hipStream_t);
+template void vec_copy<float, double>(long long, const double*, float*, hipStream_t);
+template void vec_copy<double, float>(long long, const float*, double*, hipStream_t);
+
+template <typename T>
+__global__ void k_zero(long long n, T* __restrict__ dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = T(0);
+}
+
+template <typename T>
+void vec_zero(long long n, T* dst, hipStream_t s) {
+  if (n > 0) k_zero<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, dst);
+}
+template void vec_zero<double>(long long, double*, hipStream_t);
+template void vec_zero<float>(long long, float*, hipStream_t);
+template void vec_zero<int>(long long, int*, hipStream_t);

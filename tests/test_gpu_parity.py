@@ -1,0 +1,224 @@
+"""GPU parity of the HIP library against the FP64 oracle (called through the C ABI).
+
+Tolerances: fine-level operator and RHS 1e-13 (relative to the largest entry);
+C_H / kappa_H Frobenius-relative 1e-9 when both sides solve to rtol 1e-10
+(the FP64 form's bar), sensitivities 1e-8, filter 1e-14, OC 1e-12 on
+identical inputs.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fe as ofe, grid as ogrid, homog as ohomog, objective as oobj, oc as ooc
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0) if torch.cuda.is_available() else None
+
+
+def _cells(dims, nlevels):
+    import maple_amd
+    return maple_amd.CellBatch(dims, nlevels=nlevels, device=DEV)
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(DEV)
+
+
+def _rho(r, seed, contrast=1e-9):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, 1.0, r ** 3)
+    return contrast + (1 - contrast) * x ** 3
+
+
+@pytest.mark.parametrize("r,nlev", [(8, 3), (16, 3), (32, 3), (72, 4)])
+@pytest.mark.parametrize("physics", ["elastic", "thermal"])
+def test_operator_and_rhs(r, nlev, physics):
+    print(f"[operator] r={r} {physics}", flush=True)
+    cb = _cells([r], nlev)
+    rho = _rho(r, r)
+    nd, nc = (3, 6) if physics == "elastic" else (1, 3)
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal((nc, nd, r ** 3))
+    v = cb.apply_operator(physics, _t(rho), _t(u)).cpu().numpy()
+    Ke = ofe.hexahedron(r, ofe.isotropic_elastic_tensor(1, 0.3))[0] if nd == 3 else ofe.hexahedron_scalar(r)[0]
+    for j in range(nc):
+        ref = ogrid.ebe_matvec(u[j].T.copy(), rho, Ke, (r, r, r)).T
+        assert np.abs(v[j] - ref).max() <= 1e-13 * np.abs(ref).max()
+    Fe = ofe.hexahedron(r, ofe.isotropic_elastic_tensor(1, 0.3))[1] if nd == 3 else ofe.hexahedron_scalar(r)[1]
+    F = cb.rhs(physics, _t(rho)).cpu().numpy()                 # (nc, nd, N)
+    Fo = ogrid.assemble_rhs(rho, Fe, (r, r, r))                 # (N, nd, nc)
+    assert np.abs(F - Fo.transpose(2, 1, 0)).max() <= 1e-13 * np.abs(Fo).max()
+
+
+@pytest.mark.parametrize("r,nlev,seed", [(8, 2, 0), (8, 3, 1), (16, 3, 2), (16, 4, 3)])
+def test_homogenize_parity(r, nlev, seed):
+    print(f"[homogenize] r={r} nlev={nlev}", flush=True)
+    cb = _cells([r], nlev)
+    rho = _rho(r, seed)
+    CH, kap, it = cb.homogenize(_t(rho), rtol=1e-10)
+    cs = ohomog.CellSolver((r, r, r), nlevels=3 if r >= 8 else 2, rtol=1e-12)
+    CHo, S6, _ = cs.elastic(rho)
+    ko, S3, _ = cs.thermal(rho)
+    assert np.linalg.norm(CH[0].cpu().numpy() - CHo) <= 1e-9 * np.linalg.norm(CHo)
+    assert np.linalg.norm(kap[0].cpu().numpy() - ko) <= 1e-9 * np.linalg.norm(ko)
+    w = {"stiffness": 0.4, "shear": 0.3, "thermal": 0.3}
+    J, G6, G3 = oobj.evaluate(w, CHo, ko)
+    d = cb.sensitivity(_t(G6).reshape(1, 6, 6), _t(G3).reshape(1, 3, 3)).cpu().numpy()
+    do = np.einsum("ij,eij->e", G6, S6) + np.einsum("ij,eij->e", G3, S3)
+    assert np.abs(d - do).max() <= 1e-8 * np.abs(do).max()
+    assert it.max() < 1000   # random per-voxel 1:1e9 contrast is the hardest case for point Jacobi
+
+
+def test_golden_sphere_void_32(golden_dir):
+    import os
+    g = np.load(os.path.join(golden_dir, "sphere_void_32.npz"))
+    r = 32
+    cb = _cells([r], 3)
+    rho = ooc.simp(g["x0"])
+    CH, kap, it = cb.homogenize(_t(rho), rtol=1e-10)
+    assert np.linalg.norm(CH[0].cpu().numpy() - g["CH"]) <= 1e-9 * np.linalg.norm(g["CH"])
+    assert np.linalg.norm(kap[0].cpu().numpy() - g["kappa"]) <= 1e-9 * np.linalg.norm(g["kappa"])
+    d = cb.sensitivity(_t(g["G6"]).reshape(1, 6, 6), _t(g["G3"]).reshape(1, 3, 3)).cpu().numpy()
+    assert np.abs(d - g["dJdrho"]).max() <= 1e-8 * np.abs(g["dJdrho"]).max()
+    # the plain rtol 1e-6 solve already sits far inside the 1e-5 mixed-form bar
+    CH6, _, _ = cb.homogenize(_t(rho), rtol=1e-6)
+    assert np.linalg.norm(CH6[0].cpu().numpy() - g["CH"]) <= 1e-9 * np.linalg.norm(g["CH"])
+
+
+@pytest.mark.parametrize("k", [1, 3, 5])
+def test_filter_parity(k):
+    dims = [8, 16, 12]
+    cb = _cells(dims, 2)
+    rng = np.random.default_rng(k)
+    v = rng.standard_normal(sum(d ** 3 for d in dims))
+    out = cb.filter(_t(v), k).cpu().numpy()
+    o = 0
+    for d in dims:
+        n = d ** 3
+        ref = ooc.box_filter(v[o:o + n], (d, d, d), k)
+        assert np.abs(out[o:o + n] - ref).max() <= 1e-14 * max(1.0, np.abs(ref).max())
+        o += n
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_oc_parity(seed):
+    dims = [16, 8]
+    cb = _cells(dims, 2)
+    rng = np.random.default_rng(seed)
+    n = sum(d ** 3 for d in dims)
+    x = np.clip(0.25 + rng.uniform(-0.2, 0.2, n), 1e-3, 1.0)
+    dJ = rng.uniform(0.0, 1.0, n) ** 2 * 10.0 ** rng.uniform(-3, 1)
+    vf = [0.25, 0.4]
+    xn = cb.oc_update(_t(x), _t(dJ), vf, 0.2, 1e-3).cpu().numpy()
+    o = 0
+    for i, d in enumerate(dims):
+        m = d ** 3
+        ref, lam = ooc.oc_update(x[o:o + m], dJ[o:o + m], vf[i], 0.2, 1e-3)
+        assert np.abs(xn[o:o + m] - ref).max() <= 1e-12
+        o += m
+
+
+def test_batched_mixed_sizes_equal_single():
+    dims = [8, 16, 8]
+    cb = _cells(dims, 3)
+    rhos = [_rho(d, 10 + i) for i, d in enumerate(dims)]
+    CH, kap, it = cb.homogenize(_t(np.concatenate(rhos)), rtol=1e-10)
+    for i, d in enumerate(dims):
+        single = _cells([d], 3)
+        c1, k1, _ = single.homogenize(_t(rhos[i]), rtol=1e-10)
+        assert np.linalg.norm(CH[i].cpu().numpy() - c1[0].cpu().numpy()) <= 1e-10 * np.linalg.norm(c1[0].cpu().numpy())
+        assert np.linalg.norm(kap[i].cpu().numpy() - k1[0].cpu().numpy()) <= 1e-10 * np.linalg.norm(k1[0].cpu().numpy())
+
+
+def test_closed_forms_on_gpu():
+    r = 16
+    cb = _cells([r], 3)
+    C0 = ofe.isotropic_elastic_tensor(1, 0.3)
+    CH, kap, it = cb.homogenize(_t(np.full(r ** 3, 0.7)), rtol=1e-10)
+    assert np.abs(CH[0].cpu().numpy() - 0.7 * C0).max() < 1e-14
+    assert np.abs(kap[0].cpu().numpy() - 0.7 * np.eye(3)).max() < 1e-14
+    prof = np.random.default_rng(3).uniform(0.2, 1.0, r)
+    prof[::4] = 1e-9
+    CH, kap, it = cb.homogenize(_t(np.tile(prof, r * r)), rtol=1e-12)
+    lam, mu = C0[0, 1], C0[3, 3]
+    a = lam + 2 * mu
+    h, m = 1 / np.mean(1 / prof), np.mean(prof)
+    assert abs(CH[0, 2, 2].item() - a * h) <= 1e-9 * a * h
+    assert abs(CH[0, 3, 3].item() - mu * m) <= 1e-10 * mu * m
+    assert abs(kap[0, 2, 2].item() - h) <= 1e-9 * h
+
+
+def test_properties_64():
+    """64^3 at the config-2 start: Euler identity, symmetry, translation invariance, determinism."""
+    import maple_amd
+    r = 64
+    cb = _cells([r], 4)
+    x = maple_amd.x0_uniform_noise(r, seed=0)
+    rho = ooc.simp(x)
+    CH, kap, it = cb.homogenize(_t(rho), rtol=1e-6)
+    CHn = CH[0].cpu().numpy()
+    assert np.abs(CHn - CHn.T).max() <= 1e-12 * np.abs(CHn).max()
+    # Euler: sum_e rho_e S_e(i,j) = C_H(i,j)
+    for (i, j) in [(0, 0), (0, 1), (3, 3), (2, 5)]:
+        G = np.zeros((6, 6))
+        G[i, j] = 1.0
+        s = cb.sensitivity(_t(G).reshape(1, 6, 6)).cpu().numpy()
+        assert abs(np.dot(rho, s) - CHn[i, j]) <= 1e-12 * np.abs(CHn).max()
+    CH2, kap2, _ = cb.homogenize(_t(rho), rtol=1e-6)
+    assert torch.equal(CH, CH2) and torch.equal(kap, kap2)          # bitwise run-to-run
+    sh = np.roll(rho.reshape(r, r, r), (5, -3, 17), axis=(0, 1, 2)).ravel()
+    CH3, kap3, _ = cb.homogenize(_t(sh), rtol=1e-8)
+    CH4, kap4, _ = cb.homogenize(_t(rho), rtol=1e-8)
+    assert np.linalg.norm((CH3 - CH4).cpu().numpy()) <= 1e-9 * np.linalg.norm(CH4.cpu().numpy())
+
+
+def test_errors_are_exceptions():
+    import maple_amd
+    from maple_amd._lib import NotConverged
+    r = 8
+    cb = _cells([r], 3)
+    rho = _rho(r, 5)
+    with pytest.raises(NotConverged):
+        cb.homogenize(_t(rho), rtol=1e-14, maxit=2)
+    bad = rho.copy()
+    bad[7] = np.nan
+    with pytest.raises(maple_amd._lib.MapleError):
+        cb.homogenize(_t(bad), rtol=1e-6)
+    with pytest.raises(ValueError):
+        cb.filter(_t(np.zeros(r ** 3)), 2)
+    # the context still works after the failures
+    CH, kap, it = cb.homogenize(_t(rho), rtol=1e-8)
+    assert np.isfinite(CH.cpu().numpy()).all()
+
+
+def test_ihd_dropin_autograd():
+    """The reference-shaped IHD: J(C_H, kappa) through autograd gives the oracle's dJ/dx."""
+    import maple_amd
+    r = 8
+    cfg = maple_amd.make_cfg(r, smooth_iter=[2, 2, 3])
+    ihd = maple_amd.IHD(cfg, DEV, rtol=1e-11)
+    x = torch.from_numpy(np.clip(0.3 + np.random.default_rng(2).uniform(-0.25, 0.25, r ** 3), 1e-3, 1)).to(DEV)
+    batch = maple_amd.build_unitcell_batch(DEV, r, ooc.simp(x).reshape(r, r, r).float())
+    xv = x.reshape(-1, 1).clone().requires_grad_(True)
+    rho = 1e-9 + (1 - 1e-9) * xv.clamp(1e-3, 1.0) ** 3
+    batch["rho"] = rho
+    n = batch["feature"].shape[0]
+    Ft = maple_amd.assemble_F_fast_scalar(batch["node_index"], ihd.Fe_1d, n, rho)
+    kap = ihd.TO_homo_1d(batch, ihd.Ke_1d, Ft, ihd.X0_1d, batch_size=1)
+    Fm = maple_amd.assemble_F_fast(batch["node_index"], ihd.Fe_3d, n, rho)
+    CH = ihd.TO_homo_3d(batch, ihd.Ke_3d, Fm, ihd.X0_3d, batch_size=1)
+    J = maple_amd.objective_from_spec(obj="stiffness:0.5,thermal:0.5", CH=CH, kappa=kap)
+    (g,) = torch.autograd.grad(J, xv)
+    xn = x.cpu().numpy()
+    cs = ohomog.CellSolver((r, r, r), nlevels=3, rtol=1e-12)
+    rr = ooc.simp(xn)
+    CHo, S6, _ = cs.elastic(rr)
+    ko, S3, _ = cs.thermal(rr)
+    Jo, G6, G3 = oobj.evaluate({"stiffness": 0.5, "thermal": 0.5}, CHo, ko)
+    go = (np.einsum("ij,eij->e", G6, S6) + np.einsum("ij,eij->e", G3, S3)) * ooc.simp_grad(xn)
+    assert abs(J.item() - Jo) <= 1e-9 * abs(Jo)
+    assert np.abs(g.cpu().numpy().ravel() - go).max() <= 1e-8 * np.abs(go).max()
+    # module-level drop-ins on the same inputs
+    f = maple_amd.avg_filter3d_x_periodic(torch.from_numpy(go).to(DEV).reshape(-1, 1), batch["elem_coords"], r, 3)
+    assert np.abs(f.cpu().numpy().ravel() - ooc.box_filter(go, (r, r, r), 3)).max() <= 1e-14 * np.abs(go).max()
