@@ -74,6 +74,14 @@ int maple_set_element_matrices(maple_ctx* ctx, int cell, const double* Ke24, con
  * EBE_GMG.py:858). */
 int maple_set_solver(maple_ctx* ctx, double omega, int nsmooth, int warm_start);
 
+/* Smoother of the V-cycle: MAPLE_SMOOTH_JACOBI (weighted point Jacobi; the
+ * reference's smooth_jacobi, EBE_GMG.py:395-411) or MAPLE_SMOOTH_L1_JACOBI
+ * (the diagonal replaced by the absolute row sum, convergent for every
+ * density field; the default). */
+#define MAPLE_SMOOTH_JACOBI 0
+#define MAPLE_SMOOTH_L1_JACOBI 1
+int maple_set_smoother(maple_ctx* ctx, int kind);
+
 /* Homogenize: for every cell, solve the 6 elastic unit-strain (and/or 3 thermal
  * unit-gradient) periodic cell problems K(rho) u = F(rho) by multigrid-
  * preconditioned CG to ||r|| <= max(rtol ||f||, 1e-14 ||Fe||_F sqrt(E)),

@@ -22,6 +22,8 @@ MAPLE_ELASTIC = 1
 MAPLE_THERMAL = 2
 MAPLE_PREC_FP64 = 0
 MAPLE_PREC_MIXED = 1
+MAPLE_SMOOTH_JACOBI = 0
+MAPLE_SMOOTH_L1_JACOBI = 1
 
 _P = C.c_void_p
 _D = C.c_double
@@ -39,6 +41,7 @@ SIGNATURES = {
     "maple_ctx_destroy": (_I, [_P]),
     "maple_set_element_matrices": (_I, [_P, _I, _DP, _DP, _DP, _DP, _DP, _DP]),
     "maple_set_solver": (_I, [_P, _D, _I, _I]),
+    "maple_set_smoother": (_I, [_P, _I]),
     "maple_homogenize": (_I, [_P, _P, _I, _D, _I, _P, _P, _IP, _P]),
     "maple_sensitivity": (_I, [_P, _P, _P, _P, _P]),
     "maple_filter_box": (_I, [_P, _P, _P, _I, _P]),

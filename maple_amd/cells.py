@@ -42,7 +42,8 @@ def max_levels(r: int, min_coarse: int = 4) -> int:
 
 class CellBatch:
     def __init__(self, dims: Sequence, nlevels: int = 4, device=None, precision: str = "fp64",
-                 omega: float = 0.6, nsmooth: int = 2, warm_start: bool = False):
+                 omega: Optional[float] = None, nsmooth: int = 2, warm_start: bool = False,
+                 smoother: str = "l1-jacobi"):
         self.lib = L.load()
         dl = []
         for d in dims:
@@ -69,6 +70,9 @@ class CellBatch:
         self.nelem = int(self.offsets[-1])
         self.generation = {"elastic": 0, "thermal": 0}
         self.last_iters = None
+        self.set_smoother(smoother)
+        if omega is None:
+            omega = 1.6 if smoother == "l1-jacobi" else 0.6
         self.set_solver(omega, nsmooth, warm_start)
 
     # -- lifetime ---------------------------------------------------------
@@ -100,8 +104,14 @@ class CellBatch:
         return torch.empty(*shape, dtype=torch.float64, device=self.device)
 
     # -- solver ------------------------------------------------------------
-    def set_solver(self, omega: float = 0.6, nsmooth: int = 2, warm_start: bool = False):
+    def set_solver(self, omega: float = 1.6, nsmooth: int = 2, warm_start: bool = False):
+        """Smoother weight, sweeps per V-cycle side, and start from the previous call's fields."""
         L.check(self.lib.maple_set_solver(self._h, float(omega), int(nsmooth), int(bool(warm_start))))
+
+    def set_smoother(self, kind: str = "l1-jacobi"):
+        """'jacobi' (weighted point Jacobi) or 'l1-jacobi' (absolute row sums; default)."""
+        k = {"jacobi": L.MAPLE_SMOOTH_JACOBI, "l1-jacobi": L.MAPLE_SMOOTH_L1_JACOBI}[kind]
+        L.check(self.lib.maple_set_smoother(self._h, k))
 
     def set_element_matrices(self, cell: int, Ke24=None, Fe24=None, X0_24=None, Ke8=None, Fe8=None, X0_8=None):
         def p(a):

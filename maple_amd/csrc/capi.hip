@@ -82,9 +82,10 @@ struct maple_ctx {
   std::vector<Level> lv;
   Phys ph[2];
   double* partial = nullptr;
-  double omega = 0.6;
+  double omega = 1.6;   // l1-Jacobi: the spectrum of Dinv K lies in (0, 1], any omega < 2 converges
   int nsmooth = 2;
   int warm = 0;
+  int smoother = MAPLE_SMOOTH_L1_JACOBI;
   OCState* ocst = nullptr;
   OCState* ocst_h = nullptr;
   double* oc_a = nullptr;
@@ -262,10 +263,11 @@ void apply(maple_ctx* c, Phys& P, int l, int mode, const double* rho, const doub
 
 void setup_hierarchy(maple_ctx* c, Phys& P, const double* rho, hipStream_t s) {
   const int L = c->nlevels;
-  fine_dinv<double>(c->lv[0].node, P.nd, P.Ke, rho, P.dinv[0], s);
+  const bool l1 = c->smoother == MAPLE_SMOOTH_L1_JACOBI;
+  fine_dinv<double>(c->lv[0].node, P.nd, l1, P.Ke, rho, P.dinv[0], s);
   stencil_level1<double>(c->lv[1].node, c->lv[0].node, P.nd, P.G, rho, P.S[1], s);
   for (int l = 2; l < L; ++l) stencil_galerkin<double>(c->lv[l].node, c->lv[l - 1].node, P.nd, P.S[l - 1], P.S[l], s);
-  for (int l = 1; l < L - 1; ++l) stencil_dinv<double>(c->lv[l].node, P.nd, P.S[l], P.dinv[l], s);
+  for (int l = 1; l < L - 1; ++l) stencil_dinv<double>(c->lv[l].node, P.nd, l1, P.S[l], P.dinv[l], s);
   dense_assemble_g<double>(c->lv[L - 1].node, P.ddc, P.maxn, P.nd, P.S[L - 1], P.A, P.gamma, s);
   dense_inverse(P.ddc, c->ncells, P.max_nt, P.A, P.work, s);
 }
@@ -522,6 +524,13 @@ int maple_set_solver(maple_ctx* c, double omega, int nsmooth, int warm_start) {
   c->omega = omega;
   c->nsmooth = nsmooth;
   c->warm = warm_start ? 1 : 0;
+  return 0;
+}
+
+int maple_set_smoother(maple_ctx* c, int kind) {
+  if (!c) return fail(MAPLE_EINVAL, "null context");
+  if (kind != MAPLE_SMOOTH_JACOBI && kind != MAPLE_SMOOTH_L1_JACOBI) return fail(MAPLE_EINVAL, "unknown smoother");
+  c->smoother = kind;
   return 0;
 }
 

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void k_stencil_apply(const LevelCell* __restri
                                                        const T* __restrict__ dinv, T omega,
                                                        T* __restrict__ out) {
   LevelCell c; int ci, x, y, z; long long gid;
-  if (!node_of(cells, blk_start, ncells, blockIdx.x, c, ci, x, y, z, gid)) return;
+  if (!node_of(cells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
   T acc[MAPLE_MAX_CASES][ND];
 #pragma unroll
   for (int j = 0; j < MAPLE_MAX_CASES; ++j)
@@ -106,25 +106,32 @@ template void stencil_apply<1, double>(const Launch&, int, int, const double*, c
 template void stencil_apply<3, float>(const Launch&, int, int, const float*, const float*, const float*, const float*, float, float*, double*, hipStream_t);
 template void stencil_apply<1, float>(const Launch&, int, int, const float*, const float*, const float*, const float*, float, float*, double*, hipStream_t);
 
+// point diagonal, or (l1) the absolute row sum of the assembled operator
 template <typename T>
 __global__ __launch_bounds__(256) void k_stencil_dinv(const LevelCell* __restrict__ cells,
                                                       const int* __restrict__ blk_start, int ncells,
-                                                      long long Ntot, int nd, const T* __restrict__ S,
+                                                      long long Ntot, int nd, int l1, const T* __restrict__ S,
                                                       T* __restrict__ dinv) {
   LevelCell c; int ci, x, y, z; long long gid;
-  if (!node_of(cells, blk_start, ncells, blockIdx.x, c, ci, x, y, z, gid)) return;
+  if (!node_of(cells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
   for (int i = 0; i < nd; ++i) {
-    const T dv = S[(long long)(13 * nd * nd + i * nd + i) * Ntot + gid];
-    dinv[(long long)i * Ntot + gid] = dv != T(0) ? T(1) / dv : T(1);
+    double dv = 0.0;
+    if (l1) {
+      for (int d = 0; d < 27; ++d)
+        for (int j = 0; j < nd; ++j) dv += fabs((double)S[(long long)(d * nd * nd + i * nd + j) * Ntot + gid]);
+    } else {
+      dv = (double)S[(long long)(13 * nd * nd + i * nd + i) * Ntot + gid];
+    }
+    dinv[(long long)i * Ntot + gid] = dv != 0.0 ? (T)(1.0 / dv) : T(1);
   }
 }
 
 template <typename T>
-void stencil_dinv(const Launch& L, int nd, const T* S, T* dinv, hipStream_t s) {
-  k_stencil_dinv<T><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, nd, S, dinv);
+void stencil_dinv(const Launch& L, int nd, bool l1, const T* S, T* dinv, hipStream_t s) {
+  k_stencil_dinv<T><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, nd, l1 ? 1 : 0, S, dinv);
 }
-template void stencil_dinv<double>(const Launch&, int, const double*, double*, hipStream_t);
-template void stencil_dinv<float>(const Launch&, int, const float*, float*, hipStream_t);
+template void stencil_dinv<double>(const Launch&, int, bool, const double*, double*, hipStream_t);
+template void stencil_dinv<float>(const Launch&, int, bool, const float*, float*, hipStream_t);
 
 // blockIdx.y = neighbour offset d; one thread per coarse node.
 template <int ND, typename T>
@@ -135,7 +142,7 @@ __global__ __launch_bounds__(256) void k_stencil_level1(const LevelCell* __restr
                                                         T* __restrict__ S) {
   constexpr int NE = 8 * ND;
   LevelCell c; int ci, x, y, z; long long gid;
-  if (!node_of(ccells, blk_start, ncells, blockIdx.x, c, ci, x, y, z, gid)) return;
+  if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
   const LevelCell cf = fcells[ci];
   const int d = blockIdx.y;
   const int dd[3] = {d / 9 - 1, (d / 3) % 3 - 1, d % 3 - 1};
@@ -187,7 +194,7 @@ __global__ __launch_bounds__(256) void k_stencil_galerkin(const LevelCell* __res
                                                           long long Ntot_f, const T* __restrict__ Sf,
                                                           T* __restrict__ Sc) {
   LevelCell c; int ci, x, y, z; long long gid;
-  if (!node_of(ccells, blk_start, ncells, blockIdx.x, c, ci, x, y, z, gid)) return;
+  if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
   const LevelCell cf = fcells[ci];
   const int D = blockIdx.y;
   const int Dx = D / 9 - 1, Dy = (D / 3) % 3 - 1, Dz = D % 3 - 1;
@@ -243,7 +250,7 @@ __global__ __launch_bounds__(256) void k_restrict(const LevelCell* __restrict__ 
                                                   const LevelCell* __restrict__ fcells, long long Ntot_f, int ncase,
                                                   const T* __restrict__ rf, T* __restrict__ bc) {
   LevelCell c; int ci, x, y, z; long long gid;
-  if (!node_of(ccells, blk_start, ncells, blockIdx.x, c, ci, x, y, z, gid)) return;
+  if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
   const LevelCell cf = fcells[ci];
   T acc[MAPLE_MAX_CASES][ND];
 #pragma unroll
@@ -289,7 +296,7 @@ __global__ __launch_bounds__(256) void k_prolong_add(const LevelCell* __restrict
                                                      const LevelCell* __restrict__ ccells, long long Ntot_c, int ncase,
                                                      const T* __restrict__ ec, T* __restrict__ uf) {
   LevelCell c; int ci, x, y, z; long long gid;
-  if (!node_of(fcells, blk_start, ncells, blockIdx.x, c, ci, x, y, z, gid)) return;
+  if (!node_of(fcells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
   const LevelCell cc = ccells[ci];
   const int bx = x >> 1, by = y >> 1, bz = z >> 1;
   const int rx = x & 1, ry = y & 1, rz = z & 1;

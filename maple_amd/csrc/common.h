@@ -32,6 +32,20 @@ __host__ __device__ constexpr int hex_corner(int a, int b, int c) {
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// XCD-aware block order.  MI355X dispatches consecutive workgroups round-robin
+// over its 8 XCDs, each with its own L2.  Logical block b is given to the
+// workgroup that XCD (b / chunk) runs, so every XCD works on one contiguous
+// run of logical blocks (a slab of neighbouring pencils that share halos in
+// its L2) instead of every eighth block of the whole domain.  Bijective on
+// [0, nb).
+#define MAPLE_NUM_XCD 8
+__host__ __device__ inline int xcd_swizzle(int bid, int nb) {
+  const int q = nb / MAPLE_NUM_XCD, r = nb % MAPLE_NUM_XCD;
+  const int xcd = bid % MAPLE_NUM_XCD, k = bid / MAPLE_NUM_XCD;
+  return xcd * q + (xcd < r ? xcd : r) + k;
+}
+__device__ __forceinline__ int lbid() { return xcd_swizzle((int)blockIdx.x, (int)gridDim.x); }
+
 // Block -> cell lookup: blk_start has ncells+1 entries, cell i owns blocks
 // [blk_start[i], blk_start[i+1]).
 __device__ __forceinline__ int find_cell(const int* __restrict__ blk_start, int ncells, int b) {

@@ -88,42 +88,68 @@ __device__ __forceinline__ void load_tile(const DenseCell& dc, const double* __r
   }
 }
 
-// K1: invert the pivot tile in LDS (scalar Gauss-Jordan), write it to work[cell].
+// K1: invert the pivot tile (scalar Gauss-Jordan, no pivoting: SPD), write it to
+// work[cell].  Each of the 256 threads keeps its 4 x 4 sub-block in registers;
+// per pivot only row p and column p go through LDS (double-buffered, one
+// barrier per pivot).
 __global__ __launch_bounds__(256) void k_gj_diag(const DenseCell* __restrict__ dcells, int k,
                                                  const double* __restrict__ A, double* __restrict__ work) {
   const int ci = blockIdx.x;
   const DenseCell dc = dcells[ci];
   if (k >= dc.nt) return;
-  __shared__ double M[TB][TB];
-  load_tile(dc, A, k, k, M);
-  const int nb = tile_rows(dc, k);
-  // pad the identity below the matrix so the padded pivots are 1
-  for (int r = threadIdx.x; r < TB; r += blockDim.x)
-    if (r >= nb) M[r][r] = 1.0;
-  __syncthreads();
+  __shared__ double rowb[2][TB], colb[2][TB];
   const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;   // 16 x 16 threads, 4 x 4 entries each
+  const int nb = tile_rows(dc, k);
+  double M[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = tr * 4 + a, j = tc * 4 + b;
+      // identity in the padding so the padded pivots are 1
+      M[a][b] = (i < nb && j < nb) ? A[dc.aoff + (long long)(k * TB + i) * dc.n + k * TB + j] : (i == j ? 1.0 : 0.0);
+    }
   for (int p = 0; p < TB; ++p) {
-    const double pv = 1.0 / M[p][p];
-    double nv[4][4];
+    const int buf = p & 1;
+    const int pa = p & 3, pt = p >> 2;
+    if (tr == pt) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (pa == 0) rowb[buf][tc * 4 + b] = M[0][b];
+        else if (pa == 1) rowb[buf][tc * 4 + b] = M[1][b];
+        else if (pa == 2) rowb[buf][tc * 4 + b] = M[2][b];
+        else rowb[buf][tc * 4 + b] = M[3][b];
+    }
+    if (tc == pt) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        if (pa == 0) colb[buf][tr * 4 + a] = M[a][0];
+        else if (pa == 1) colb[buf][tr * 4 + a] = M[a][1];
+        else if (pa == 2) colb[buf][tr * 4 + a] = M[a][2];
+        else colb[buf][tr * 4 + a] = M[a][3];
+    }
+    __syncthreads();
+    const double pv = 1.0 / rowb[buf][p];
+    double rj[4], ci4[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) rj[b] = rowb[buf][tc * 4 + b];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) ci4[a] = colb[buf][tr * 4 + a];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int i = tr * 4 + a, j = tc * 4 + b;
-        const double mij = M[i][j], mip = M[i][p], mpj = M[p][j];
-        if (i == p && j == p) nv[a][b] = pv;
-        else if (i == p) nv[a][b] = mpj * pv;
-        else if (j == p) nv[a][b] = -mip * pv;
-        else nv[a][b] = mij - mip * mpj * pv;
+        if (i == p && j == p) M[a][b] = pv;
+        else if (i == p) M[a][b] = rj[b] * pv;
+        else if (j == p) M[a][b] = -ci4[a] * pv;
+        else M[a][b] = fma(-ci4[a] * pv, rj[b], M[a][b]);
       }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) M[tr * 4 + a][tc * 4 + b] = nv[a][b];
-    __syncthreads();
   }
-  for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) work[(long long)ci * TB * TB + e] = M[e / TB][e % TB];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) work[(long long)ci * TB * TB + (tr * 4 + a) * TB + tc * 4 + b] = M[a][b];
 }
 
 // C(64x64) = X * Y with X, Y in LDS; each thread 4x4 outputs.
@@ -234,41 +260,36 @@ void dense_inverse(const DenseCell* dcells, int ncells, int max_nt, double* A, d
   }
 }
 
-// x_j = Ainv b_j for every case; one wavefront per matrix row.
+// x_j = Ainv b_j for every case; one block of 256 threads per matrix row.
 template <int ND, typename T>
 __global__ __launch_bounds__(256) void k_dense_apply(const DenseCell* __restrict__ dcells, long long Ntot,
                                                      int ncase, const double* __restrict__ Ainv,
                                                      const T* __restrict__ b, T* __restrict__ x) {
   const int ci = blockIdx.y;
   const DenseCell dc = dcells[ci];
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x;
   if (row >= dc.n) return;
   double acc[MAPLE_MAX_CASES];
 #pragma unroll
   for (int j = 0; j < MAPLE_MAX_CASES; ++j) acc[j] = 0.0;
   const double* Ar = Ainv + dc.aoff + (long long)row * dc.n;
-  for (int col = lane; col < dc.n; col += 64) {
+  for (int col = threadIdx.x; col < dc.n; col += 256) {
     const double a = Ar[col];
     const int m = col / ND, i = col - m * ND;
 #pragma unroll
     for (int j = 0; j < MAPLE_MAX_CASES; ++j)
       if (j < ncase) acc[j] = fma(a, (double)b[(long long)(j * ND + i) * Ntot + dc.voff + m], acc[j]);
   }
+  __shared__ double red[MAPLE_MAX_CASES];
+  block_sum<MAPLE_MAX_CASES>(acc, red);
   const int m = row / ND, i = row - m * ND;
-#pragma unroll
-  for (int j = 0; j < MAPLE_MAX_CASES; ++j) {
-    if (j < ncase) {
-      const double v = wave_sum(acc[j]);
-      if (lane == 0) x[(long long)(j * ND + i) * Ntot + dc.voff + m] = (T)v;
-    }
-  }
+  if (threadIdx.x < ncase) x[(long long)(threadIdx.x * ND + i) * Ntot + dc.voff + m] = (T)red[threadIdx.x];
 }
 
 template <typename T>
 void dense_apply(const Launch& Lc, const DenseCell* dcells, int max_n, int nd, int ncase, const double* Ainv,
                  const T* b, T* x, hipStream_t s) {
-  dim3 grid((max_n + 3) / 4, Lc.ncells);
+  dim3 grid(max_n, Lc.ncells);
   if (nd == 3) k_dense_apply<3, T><<<grid, 256, 0, s>>>(dcells, Lc.Ntot, ncase, Ainv, b, x);
   else k_dense_apply<1, T><<<grid, 256, 0, s>>>(dcells, Lc.Ntot, ncase, Ainv, b, x);
 }

@@ -24,9 +24,9 @@ __global__ __launch_bounds__(256) void k_element_energy(Launch L, int ncase, con
                                                         double* __restrict__ partial) {
   constexpr int NE = 8 * ND;
   constexpr int NU = MAPLE_MAX_CASES * (MAPLE_MAX_CASES + 1) / 2;
-  const int ci = find_cell(L.blk_start, L.ncells, blockIdx.x);
+  const int ci = find_cell(L.blk_start, L.ncells, lbid());
   const LevelCell c = L.cells[ci];
-  const long long n = (long long)(blockIdx.x - L.blk_start[ci]) * 256 + threadIdx.x;
+  const long long n = (long long)(lbid() - L.blk_start[ci]) * 256 + threadIdx.x;
   const bool ok = n < (long long)c.nx * c.ny * c.nz;
   Ke += (long long)ci * NE * NE;
   X0 += (long long)ci * NE * ncase;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_element_energy(Launch L, int ncase, con
   }
   __syncthreads();
   if (threadIdx.x < nu)
-    partial[(long long)blockIdx.x * NU + threadIdx.x] =
+    partial[(long long)lbid() * NU + threadIdx.x] =
         red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
 }
 
@@ -127,9 +127,9 @@ __global__ __launch_bounds__(256) void k_sensitivity(Launch L, int nc3, const do
                                                      const double* __restrict__ G6, int nc1,
                                                      const double* __restrict__ Se1, const double* __restrict__ G3,
                                                      double* __restrict__ out) {
-  const int ci = find_cell(L.blk_start, L.ncells, blockIdx.x);
+  const int ci = find_cell(L.blk_start, L.ncells, lbid());
   const LevelCell c = L.cells[ci];
-  const long long n = (long long)(blockIdx.x - L.blk_start[ci]) * 256 + threadIdx.x;
+  const long long n = (long long)(lbid() - L.blk_start[ci]) * 256 + threadIdx.x;
   if (n >= (long long)c.nx * c.ny * c.nz) return;
   const long long e = c.off + n;
   double v = 0.0;
@@ -155,9 +155,9 @@ void sensitivity(const Launch& L, int nc3, const double* Se3, const double* G6, 
 
 __global__ __launch_bounds__(256) void k_box_filter(Launch L, int k, const double* __restrict__ in,
                                                     double* __restrict__ out) {
-  const int ci = find_cell(L.blk_start, L.ncells, blockIdx.x);
+  const int ci = find_cell(L.blk_start, L.ncells, lbid());
   const LevelCell c = L.cells[ci];
-  const long long n = (long long)(blockIdx.x - L.blk_start[ci]) * 256 + threadIdx.x;
+  const long long n = (long long)(lbid() - L.blk_start[ci]) * 256 + threadIdx.x;
   if (n >= (long long)c.nx * c.ny * c.nz) return;
   const int z = (int)(n % c.nz), y = (int)((n / c.nz) % c.ny), x = (int)(n / ((long long)c.nz * c.ny));
   const int h = k / 2;
@@ -181,44 +181,42 @@ void box_filter(const Launch& L, int k, const double* in, double* out, hipStream
 
 // ---- OC ----------------------------------------------------------------------
 
-// Fill the 63 midpoints of the depth-6 subtree rooted at bracket (l1, l2),
-// heap order: node t has children 2t (l2 <- lam_t) and 2t+1 (l1 <- lam_t).
-__device__ void oc_fill_tree(OCState& s) {
-  double lo[64], hi[64];
-  lo[1] = s.l1;
-  hi[1] = s.l2;
-  for (int t = 1; t < 64; ++t) {
-    const double lam = 0.5 * (lo[t] + hi[t]);
-    s.lam[t] = lam;
-    s.c[t] = 1.0 / sqrt(lam);
-    if (t < 32) {
-      lo[2 * t] = lo[t];
-      hi[2 * t] = lam;
-      lo[2 * t + 1] = lam;
-      hi[2 * t + 1] = hi[t];
-    }
+// Midpoint of heap node t (1..63) of the depth-6 subtree rooted at bracket (l1, l2):
+// node t has children 2t (l2 <- lam_t) and 2t+1 (l1 <- lam_t); walking the bits of
+// t from the top reproduces the sequential bisection's arithmetic exactly.
+__device__ __forceinline__ double oc_node_lam(double l1, double l2, int t) {
+  int depth = 31 - __clz(t);
+  double lo = l1, hi = l2;
+  for (int lev = depth - 1; lev >= 0; --lev) {
+    const double lam = 0.5 * (lo + hi);
+    if ((t >> lev) & 1) lo = lam; else hi = lam;
   }
-  s.lam[0] = s.c[0] = 0.0;
+  return 0.5 * (lo + hi);
 }
 
-__global__ void k_oc_start(int ncells, OCState* __restrict__ st) {
-  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ci >= ncells) return;
-  OCState& s = st[ci];
-  s.l1 = 1e-12;
-  s.l2 = 1e6;
-  s.last_lam = 0.0;
-  oc_fill_tree(s);
+// one wavefront per cell: lane t fills midpoint t
+__global__ __launch_bounds__(64) void k_oc_start(OCState* __restrict__ st) {
+  OCState& s = st[blockIdx.x];
+  const int t = threadIdx.x;
+  const double l1 = 1e-12, l2 = 1e6;
+  const double lam = t == 0 ? 0.0 : oc_node_lam(l1, l2, t);
+  s.lam[t] = lam;
+  s.c[t] = t == 0 ? 0.0 : 1.0 / sqrt(lam);
+  if (t == 0) {
+    s.l1 = l1;
+    s.l2 = l2;
+    s.last_lam = 0.0;
+  }
 }
 
-void oc_start(int ncells, OCState* st, hipStream_t s) { k_oc_start<<<(ncells + 63) / 64, 64, 0, s>>>(ncells, st); }
+void oc_start(int ncells, OCState* st, hipStream_t s) { k_oc_start<<<ncells, 64, 0, s>>>(st); }
 
 // a = x sqrt(max(dJ, 1e-12)) (the lambda-free factor of x sqrt(dJ/lambda))
 __global__ __launch_bounds__(256) void k_oc_prepare(Launch L, const double* __restrict__ x,
                                                     const double* __restrict__ dJ, double* __restrict__ a) {
-  const int ci = find_cell(L.blk_start, L.ncells, blockIdx.x);
+  const int ci = find_cell(L.blk_start, L.ncells, lbid());
   const LevelCell c = L.cells[ci];
-  const long long n = (long long)(blockIdx.x - L.blk_start[ci]) * 256 + threadIdx.x;
+  const long long n = (long long)(lbid() - L.blk_start[ci]) * 256 + threadIdx.x;
   if (n >= (long long)c.nx * c.ny * c.nz) return;
   const long long e = c.off + n;
   a[e] = x[e] * sqrt(fmax(dJ[e], 1e-12));
@@ -238,11 +236,11 @@ __device__ __forceinline__ double oc_value(double a, double x, double c, double 
 __global__ __launch_bounds__(256) void k_oc_pass(Launch L, const OCState* __restrict__ st,
                                                  const double* __restrict__ x, const double* __restrict__ a,
                                                  double* __restrict__ partial) {
-  const int ci = find_cell(L.blk_start, L.ncells, blockIdx.x);
+  const int ci = find_cell(L.blk_start, L.ncells, lbid());
   const LevelCell c = L.cells[ci];
   const OCState& s = st[ci];
   const long long nn = (long long)c.nx * c.ny * c.nz;
-  const long long n0 = (long long)(blockIdx.x - L.blk_start[ci]) * 1024;
+  const long long n0 = (long long)(lbid() - L.blk_start[ci]) * 1024;
   double acc[64];
 #pragma unroll
   for (int t = 0; t < 64; ++t) acc[t] = 0.0;
@@ -264,7 +262,7 @@ __global__ __launch_bounds__(256) void k_oc_pass(Launch L, const OCState* __rest
   }
   __syncthreads();
   if (threadIdx.x < 64)
-    partial[(long long)blockIdx.x * 64 + threadIdx.x] =
+    partial[(long long)lbid() * 64 + threadIdx.x] =
         threadIdx.x == 0 ? 0.0 : red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
 }
 
@@ -272,47 +270,60 @@ void oc_pass(const Launch& L, const OCState* st, const double* x, const double* 
   k_oc_pass<<<L.nblocks, 256, 0, s>>>(L, st, x, a, partial);
 }
 
-// one wavefront per cell: means of the 63 candidates, then walk `depth` levels
-__global__ __launch_bounds__(64) void k_oc_walk(const int* __restrict__ blk_start, const LevelCell* __restrict__ cells,
-                                                int depth, const double* __restrict__ partial,
-                                                OCState* __restrict__ st) {
+// one block of 1024 per cell: the 63 candidate means (16 slices of blocks per
+// candidate, reduced in a fixed order), one lane walks `depth` levels, then the
+// 64 lanes fill the next subtree.
+__global__ __launch_bounds__(1024) void k_oc_walk(const int* __restrict__ blk_start, const LevelCell* __restrict__ cells,
+                                                  int depth, const double* __restrict__ partial,
+                                                  OCState* __restrict__ st) {
   const int ci = blockIdx.x;
   const LevelCell c = cells[ci];
   const int b0 = blk_start[ci], b1 = blk_start[ci + 1];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x & 63, sl = threadIdx.x >> 6;
   double sum = 0.0;
-  for (int b = b0; b < b1; ++b) sum += partial[(long long)b * 64 + t];
-  __shared__ double mean[64];
-  mean[t] = sum / (double)((long long)c.nx * c.ny * c.nz);
+  for (int b = b0 + sl; b < b1; b += 16) sum += partial[(long long)b * 64 + t];
+  __shared__ double part[16][64];
+  __shared__ double bracket[2];
+  part[sl][t] = sum;
   __syncthreads();
-  if (t == 0) {
-    OCState& s = st[ci];
+  OCState& s = st[ci];
+  if (threadIdx.x == 0) {
+    const double nn = (double)((long long)c.nx * c.ny * c.nz);
     int node = 1;
     double l1 = s.l1, l2 = s.l2, lam = s.last_lam;
     for (int lev = 0; lev < depth; ++lev) {
+      double tot = 0.0;
+      for (int k = 0; k < 16; ++k) tot += part[k][node];
       lam = s.lam[node];
-      const bool above = mean[node] > s.volfrac;
+      const bool above = tot / nn > s.volfrac;
       if (above) l1 = lam; else l2 = lam;
       node = 2 * node + (above ? 1 : 0);
     }
     s.l1 = l1;
     s.l2 = l2;
     s.last_lam = lam;
-    oc_fill_tree(s);
+    bracket[0] = l1;
+    bracket[1] = l2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const double lm = t == 0 ? 0.0 : oc_node_lam(bracket[0], bracket[1], t);
+    s.lam[t] = lm;
+    s.c[t] = t == 0 ? 0.0 : 1.0 / sqrt(lm);
   }
 }
 
 void oc_walk(const int* blk_start, const LevelCell* cells, int ncells, int depth, const double* partial, OCState* st,
              hipStream_t s) {
-  k_oc_walk<<<ncells, 64, 0, s>>>(blk_start, cells, depth, partial, st);
+  k_oc_walk<<<ncells, 1024, 0, s>>>(blk_start, cells, depth, partial, st);
 }
 
 __global__ __launch_bounds__(256) void k_oc_final(Launch L, const OCState* __restrict__ st,
                                                   const double* __restrict__ x, const double* __restrict__ a,
                                                   double* __restrict__ xnew) {
-  const int ci = find_cell(L.blk_start, L.ncells, blockIdx.x);
+  const int ci = find_cell(L.blk_start, L.ncells, lbid());
   const LevelCell c = L.cells[ci];
-  const long long n = (long long)(blockIdx.x - L.blk_start[ci]) * 256 + threadIdx.x;
+  const long long n = (long long)(lbid() - L.blk_start[ci]) * 256 + threadIdx.x;
   if (n >= (long long)c.nx * c.ny * c.nz) return;
   const OCState& s = st[ci];
   const long long e = c.off + n;

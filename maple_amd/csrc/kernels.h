@@ -23,7 +23,7 @@ void fine_apply(const Launch& L, int mode, int ncase, const T* Ke, const T* rho,
                 const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s);
 void fine_rhs(const Launch& L, int nd, int ncase, const double* Fe, const double* rho, double* F, hipStream_t s);
 template <typename T>
-void fine_dinv(const Launch& L, int nd, const double* Ke, const double* rho, T* dinv, hipStream_t s);
+void fine_dinv(const Launch& L, int nd, bool l1, const double* Ke, const double* rho, T* dinv, hipStream_t s);
 
 // ---- coarse levels (coarse.hip) --------------------------------------------
 // Stencil S of a level: S[(d*ND*ND + i*ND + j)*Ntot + n] is the (i,j) entry of the
@@ -32,7 +32,7 @@ template <int ND, typename T>
 void stencil_apply(const Launch& L, int mode, int ncase, const T* S, const T* u, const T* f,
                    const T* dinv, T omega, T* out, double* partial, hipStream_t s);
 template <typename T>
-void stencil_dinv(const Launch& L, int nd, const T* S, T* dinv, hipStream_t s);
+void stencil_dinv(const Launch& L, int nd, bool l1, const T* S, T* dinv, hipStream_t s);
 // level-1 stencil straight from the voxel densities: block(N, d) =
 // sum over coarse elements E containing N and N+d of sum_c rho_child(E,c) G_c[..]
 template <typename T>

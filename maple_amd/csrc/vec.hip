@@ -30,9 +30,9 @@ void reduce_partials(const int* blk_start, int ncells, int ncase, const double* 
 }
 
 __device__ __forceinline__ bool vnode(const Launch& L, int& ci, long long& gid) {
-  ci = find_cell(L.blk_start, L.ncells, blockIdx.x);
+  ci = find_cell(L.blk_start, L.ncells, lbid());
   const LevelCell c = L.cells[ci];
-  const long long n = (long long)(blockIdx.x - L.blk_start[ci]) * 256 + threadIdx.x;
+  const long long n = (long long)(lbid() - L.blk_start[ci]) * 256 + threadIdx.x;
   if (n >= (long long)c.nx * c.ny * c.nz) return false;
   gid = c.off + n;
   return true;
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void k_dot(Launch L, int nd, int ncase, const 
         const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
         v[0] += (double)a[idx] * (double)b[idx];
       }
-    block_sum<1>(v, partial + (long long)blockIdx.x * MAPLE_MAX_CASES + j);
+    block_sum<1>(v, partial + (long long)lbid() * MAPLE_MAX_CASES + j);
   }
 }
 
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void k_update_ur(Launch L, int nd, int ncase, 
         v[0] += rn * rn;
       }
     }
-    block_sum<1>(v, partial + (long long)blockIdx.x * MAPLE_MAX_CASES + j);
+    block_sum<1>(v, partial + (long long)lbid() * MAPLE_MAX_CASES + j);
   }
 }
 

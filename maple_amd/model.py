@@ -58,7 +58,7 @@ class _Homogenize(torch.autograd.Function):
 
 class IHD(nn.Module):
     def __init__(self, cfg, device, *, rtol: float = 1e-6, maxit: int = 1000, nlevels: Optional[int] = None,
-                 precision: str = "fp64", warm_start: bool = False, omega: float = 0.6, nsmooth: int = 2,
+                 precision: str = "fp64", warm_start: bool = False, omega: Optional[float] = None, nsmooth: int = 2,
                  E0: float = 1.0, nu: float = 0.3, kappa0: float = 1.0):
         super().__init__()
         self.cfg = cfg
