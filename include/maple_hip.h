@@ -35,7 +35,9 @@ extern "C" {
 #define MAPLE_THERMAL 2
 
 #define MAPLE_PREC_FP64 0      /* plain method: every kernel in FP64 */
-#define MAPLE_PREC_MIXED 1     /* FP32 inner solve inside FP64 iterative refinement */
+#define MAPLE_PREC_MIXED 1     /* FP64 CG (residual, operator, dot products, C_H) refining with an
+                                  FP32 multigrid preconditioner (level-0 element contraction,
+                                  stencils, smoothing); the coarsest inverse stays FP64 */
 
 typedef struct maple_ctx maple_ctx;
 
@@ -81,6 +83,11 @@ int maple_set_solver(maple_ctx* ctx, double omega, int nsmooth, int warm_start);
 #define MAPLE_SMOOTH_JACOBI 0
 #define MAPLE_SMOOTH_L1_JACOBI 1
 int maple_set_smoother(maple_ctx* ctx, int kind);
+
+/* CG variant: 0 = Fletcher-Reeves beta, 1 = flexible (Polak-Ribiere) beta, which
+ * stays valid for a preconditioner that is not exactly symmetric (one extra dot
+ * product per iteration), -1 = flexible in MAPLE_PREC_MIXED only (default). */
+int maple_set_cg(maple_ctx* ctx, int flexible);
 
 /* Homogenize: for every cell, solve the 6 elastic unit-strain (and/or 3 thermal
  * unit-gradient) periodic cell problems K(rho) u = F(rho) by multigrid-
@@ -132,8 +139,9 @@ int maple_set_fields(maple_ctx* ctx, int physics, const double* u_dev, void* str
 long long maple_num_elements(const maple_ctx* ctx);
 
 /* Kernel timing with HIP events on the call's stream around every launch of
- * the elastic level-0 operator kernels (enable, then read: count of launches
- * and their summed duration in ms; reading resets). */
+ * the level-0 operator kernels (enable, then read: count of launches and their
+ * summed duration in ms; reading resets).  kind 0: FP64 elastic, 1: FP64
+ * thermal, 2: FP32 elastic (mixed form's preconditioner). */
 int maple_timing_enable(maple_ctx* ctx, int enable);
 int maple_timing_read(maple_ctx* ctx, int kind, long long* count, double* total_ms);
 

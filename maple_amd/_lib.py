@@ -42,6 +42,7 @@ SIGNATURES = {
     "maple_set_element_matrices": (_I, [_P, _I, _DP, _DP, _DP, _DP, _DP, _DP]),
     "maple_set_solver": (_I, [_P, _D, _I, _I]),
     "maple_set_smoother": (_I, [_P, _I]),
+    "maple_set_cg": (_I, [_P, _I]),
     "maple_homogenize": (_I, [_P, _P, _I, _D, _I, _P, _P, _IP, _P]),
     "maple_sensitivity": (_I, [_P, _P, _P, _P, _P]),
     "maple_filter_box": (_I, [_P, _P, _P, _I, _P]),

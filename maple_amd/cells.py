@@ -113,6 +113,10 @@ class CellBatch:
         k = {"jacobi": L.MAPLE_SMOOTH_JACOBI, "l1-jacobi": L.MAPLE_SMOOTH_L1_JACOBI}[kind]
         L.check(self.lib.maple_set_smoother(self._h, k))
 
+    def set_cg(self, flexible: Optional[bool] = None):
+        """Flexible (Polak-Ribiere) CG on/off; None = only in the mixed form."""
+        L.check(self.lib.maple_set_cg(self._h, -1 if flexible is None else int(bool(flexible))))
+
     def set_element_matrices(self, cell: int, Ke24=None, Fe24=None, X0_24=None, Ke8=None, Fe8=None, X0_8=None):
         def p(a):
             if a is None:

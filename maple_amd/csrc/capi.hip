@@ -11,6 +11,7 @@
 #include "../../include/maple_hip.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -50,11 +51,22 @@ struct Level {
   Launch node, fine, oc;
 };
 
+// Multigrid hierarchy and V-cycle work vectors in precision T (FP64 for the
+// plain method, FP32 for the mixed form's preconditioner).
+template <typename T>
+struct VC {
+  std::vector<T*> S, dinv, X, B, R, W;
+  T* Ke = nullptr;    // level-0 element matrices, per cell
+  T* rho = nullptr;   // level-0 densities (the FP64 form uses the caller's array)
+};
+
 struct Phys {
   int nd = 0, nc = 0;
   double *Ke = nullptr, *Fe = nullptr, *X0 = nullptr, *G = nullptr;   // per cell on device
   std::vector<double> hKe, hFe, hX0;                                   // per cell on host
-  std::vector<double*> S, dinv, X, B, R, T;
+  VC<double> v64;
+  VC<float> v32;
+  double* z64 = nullptr;   // mixed form: the FP32 preconditioner's output, widened
   double *u = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *f = nullptr;
   double *A = nullptr, *work = nullptr, *gamma = nullptr;
   std::vector<DenseCell> hdc;
@@ -86,12 +98,13 @@ struct maple_ctx {
   int nsmooth = 2;
   int warm = 0;
   int smoother = MAPLE_SMOOTH_L1_JACOBI;
+  int flexible = -1;   // -1: flexible CG in the mixed form only
   OCState* ocst = nullptr;
   OCState* ocst_h = nullptr;
   double* oc_a = nullptr;
   hipEvent_t poll[2] = {nullptr, nullptr};
   bool timing = false;
-  Timer timers[2];
+  Timer timers[3];   // 0: FP64 elastic level-0 operator, 1: FP64 thermal, 2: FP32 elastic
   std::vector<void*> allocs;
   std::vector<void*> host_allocs;
 };
@@ -140,6 +153,10 @@ int upload_phys_constants(maple_ctx* c, Phys& P) {
   HIPCHK(hipMemcpy(P.Fe, P.hFe.data(), P.hFe.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P.X0, P.hX0.data(), P.hX0.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(P.G, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (P.v32.Ke) {
+    std::vector<float> k32(P.hKe.begin(), P.hKe.end());
+    HIPCHK(hipMemcpy(P.v32.Ke, k32.data(), k32.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
   // absolute residual floor per cell: (1e-14 ||Fe||_F sqrt(E))^2
   std::vector<double> at(c->ncells);
   for (int i = 0; i < c->ncells; ++i) {
@@ -162,23 +179,35 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
   TRY(dalloc(c, &P.X0, (size_t)c->ncells * ne * nc));
   TRY(dalloc(c, &P.G, (size_t)c->ncells * 8 * ne * ne));
   TRY(dalloc(c, &P.atol2, (size_t)c->ncells));
-  P.S.assign(L, nullptr);
-  P.dinv.assign(L, nullptr);
-  P.X.assign(L, nullptr);
-  P.B.assign(L, nullptr);
-  P.R.assign(L, nullptr);
-  P.T.assign(L, nullptr);
-  for (int l = 0; l < L; ++l) {
-    const long long N = c->lv[l].Ntot;
-    const size_t nv = (size_t)nc * nd * N;
-    if (l >= 1) TRY(dalloc(c, &P.S[l], (size_t)27 * nd * nd * N));
-    if (l < L - 1) {
-      TRY(dalloc(c, &P.dinv[l], (size_t)nd * N));
-      TRY(dalloc(c, &P.R[l], nv));
-      TRY(dalloc(c, &P.T[l], nv));
+  auto alloc_vc = [&](auto& V) -> int {
+    using TT = typename std::remove_pointer<typename std::decay<decltype(V.S[0])>::type>::type;
+    V.S.assign(L, nullptr);
+    V.dinv.assign(L, nullptr);
+    V.X.assign(L, nullptr);
+    V.B.assign(L, nullptr);
+    V.R.assign(L, nullptr);
+    V.W.assign(L, nullptr);
+    for (int l = 0; l < L; ++l) {
+      const long long N = c->lv[l].Ntot;
+      const size_t nv = (size_t)nc * nd * N;
+      if (l >= 1) TRY(dalloc(c, (TT**)&V.S[l], (size_t)27 * nd * nd * N));
+      if (l < L - 1) {
+        TRY(dalloc(c, (TT**)&V.dinv[l], (size_t)nd * N));
+        TRY(dalloc(c, (TT**)&V.R[l], nv));
+        TRY(dalloc(c, (TT**)&V.W[l], nv));
+      }
+      TRY(dalloc(c, (TT**)&V.X[l], nv));
+      TRY(dalloc(c, (TT**)&V.B[l], nv));
     }
-    TRY(dalloc(c, &P.X[l], nv));
-    if (l >= 1) TRY(dalloc(c, &P.B[l], nv));
+    return 0;
+  };
+  if (c->prec == MAPLE_PREC_FP64) {
+    TRY(alloc_vc(P.v64));
+  } else {
+    TRY(alloc_vc(P.v32));
+    TRY(dalloc(c, &P.v32.Ke, (size_t)c->ncells * ne * ne));
+    TRY(dalloc(c, &P.v32.rho, (size_t)c->lv[0].Ntot));
+    TRY(dalloc(c, &P.z64, (size_t)nc * nd * c->lv[0].Ntot));
   }
   const size_t n0 = (size_t)nc * nd * c->lv[0].Ntot;
   TRY(dalloc(c, &P.u, n0));
@@ -243,58 +272,97 @@ void timed_end(maple_ctx* c, int kind, hipStream_t s) {
   t.used += 2;
 }
 
-void apply(maple_ctx* c, Phys& P, int l, int mode, const double* rho, const double* u, const double* f, double* out,
-           double* partial, hipStream_t s) {
+// FP64 level-0 operator for the CG matvec and residuals (any precision mode).
+void apply_fine64(maple_ctx* c, Phys& P, int mode, const double* rho, const double* u, const double* f, double* out,
+                  double* partial, hipStream_t s) {
+  const int kind = P.nd == 3 ? 0 : 1;
+  timed_begin(c, kind, s);
+  if (P.nd == 3)
+    fine_apply<3, double>(c->lv[0].fine, mode, P.nc, P.Ke, rho, u, f, P.v64.dinv.empty() ? nullptr : P.v64.dinv[0],
+                          c->omega, out, partial, s);
+  else
+    fine_apply<1, double>(c->lv[0].fine, mode, P.nc, P.Ke, rho, u, f, P.v64.dinv.empty() ? nullptr : P.v64.dinv[0],
+                          c->omega, out, partial, s);
+  timed_end(c, kind, s);
+}
+
+// Operator of level l of hierarchy V (mode 2: residual, 3: Jacobi sweep).
+template <typename T>
+void apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const T* u, const T* f, T* out, hipStream_t s) {
   if (l == 0) {
-    const int kind = P.nd == 3 ? 0 : 1;
-    timed_begin(c, kind, s);
+    const int kind = P.nd == 3 ? (sizeof(T) == 8 ? 0 : 2) : (sizeof(T) == 8 ? 1 : -1);
+    if (kind >= 0) timed_begin(c, kind, s);
     if (P.nd == 3)
-      fine_apply<3, double>(c->lv[0].fine, mode, P.nc, P.Ke, rho, u, f, P.dinv[0], c->omega, out, partial, s);
+      fine_apply<3, T>(c->lv[0].fine, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
     else
-      fine_apply<1, double>(c->lv[0].fine, mode, P.nc, P.Ke, rho, u, f, P.dinv[0], c->omega, out, partial, s);
-    timed_end(c, kind, s);
+      fine_apply<1, T>(c->lv[0].fine, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+    if (kind >= 0) timed_end(c, kind, s);
   } else {
     if (P.nd == 3)
-      stencil_apply<3, double>(c->lv[l].node, mode, P.nc, P.S[l], u, f, P.dinv[l], c->omega, out, nullptr, s);
+      stencil_apply<3, T>(c->lv[l].node, mode, P.nc, V.S[l], u, f, V.dinv[l], (T)c->omega, out, nullptr, s);
     else
-      stencil_apply<1, double>(c->lv[l].node, mode, P.nc, P.S[l], u, f, P.dinv[l], c->omega, out, nullptr, s);
+      stencil_apply<1, T>(c->lv[l].node, mode, P.nc, V.S[l], u, f, V.dinv[l], (T)c->omega, out, nullptr, s);
   }
 }
 
-void setup_hierarchy(maple_ctx* c, Phys& P, const double* rho, hipStream_t s) {
+// Hierarchy of one density field: Jacobi diagonals, Galerkin stencils, dense coarsest inverse.
+template <typename T>
+void setup_t(maple_ctx* c, Phys& P, VC<T>& V, const double* rho, hipStream_t s) {
   const int L = c->nlevels;
   const bool l1 = c->smoother == MAPLE_SMOOTH_L1_JACOBI;
-  fine_dinv<double>(c->lv[0].node, P.nd, l1, P.Ke, rho, P.dinv[0], s);
-  stencil_level1<double>(c->lv[1].node, c->lv[0].node, P.nd, P.G, rho, P.S[1], s);
-  for (int l = 2; l < L; ++l) stencil_galerkin<double>(c->lv[l].node, c->lv[l - 1].node, P.nd, P.S[l - 1], P.S[l], s);
-  for (int l = 1; l < L - 1; ++l) stencil_dinv<double>(c->lv[l].node, P.nd, l1, P.S[l], P.dinv[l], s);
-  dense_assemble_g<double>(c->lv[L - 1].node, P.ddc, P.maxn, P.nd, P.S[L - 1], P.A, P.gamma, s);
+  fine_dinv<T>(c->lv[0].node, P.nd, l1, P.Ke, rho, V.dinv[0], s);
+  stencil_level1<T>(c->lv[1].node, c->lv[0].node, P.nd, P.G, rho, V.S[1], s);
+  for (int l = 2; l < L; ++l) stencil_galerkin<T>(c->lv[l].node, c->lv[l - 1].node, P.nd, V.S[l - 1], V.S[l], s);
+  for (int l = 1; l < L - 1; ++l) stencil_dinv<T>(c->lv[l].node, P.nd, l1, V.S[l], V.dinv[l], s);
+  dense_assemble_g<T>(c->lv[L - 1].node, P.ddc, P.maxn, P.nd, V.S[L - 1], P.A, P.gamma, s);
   dense_inverse(P.ddc, c->ncells, P.max_nt, P.A, P.work, s);
 }
 
+void setup_hierarchy(maple_ctx* c, Phys& P, const double* rho, hipStream_t s) {
+  if (c->prec == MAPLE_PREC_FP64) {
+    P.v64.Ke = P.Ke;
+    P.v64.rho = const_cast<double*>(rho);
+    setup_t<double>(c, P, P.v64, rho, s);
+  } else {
+    vec_copy<float, double>(c->lv[0].Ntot, rho, P.v32.rho, s);
+    setup_t<float>(c, P, P.v32, rho, s);
+  }
+}
+
 // z = M b at level l (symmetric V-cycle, zero start); returns the buffer holding the result.
-double* vcycle(maple_ctx* c, Phys& P, int l, const double* b, const double* rho, hipStream_t s) {
+template <typename T>
+T* vcycle_t(maple_ctx* c, Phys& P, VC<T>& V, int l, const T* b, hipStream_t s) {
   const int L = c->nlevels;
   if (l == L - 1) {
-    dense_apply<double>(c->lv[l].node, P.ddc, P.maxn, P.nd, P.nc, P.A, b, P.X[l], s);
-    return P.X[l];
+    dense_apply<T>(c->lv[l].node, P.ddc, P.maxn, P.nd, P.nc, P.A, b, V.X[l], s);
+    return V.X[l];
   }
-  double* x = P.X[l];
-  double* t = P.T[l];
-  vec_scale_dinv<double>(c->lv[l].node, P.nd, P.nc, c->omega, P.dinv[l], b, x, s);
+  T* x = V.X[l];
+  T* t = V.W[l];
+  vec_scale_dinv<T>(c->lv[l].node, P.nd, P.nc, (T)c->omega, V.dinv[l], b, x, s);
   for (int k = 1; k < c->nsmooth; ++k) {
-    apply(c, P, l, 3, rho, x, b, t, nullptr, s);
+    apply_t<T>(c, P, V, l, 3, x, b, t, s);
     std::swap(x, t);
   }
-  apply(c, P, l, 2, rho, x, b, P.R[l], nullptr, s);
-  restrict_pt<double>(c->lv[l + 1].node, c->lv[l].node, P.nd, P.nc, P.R[l], P.B[l + 1], s);
-  double* e = vcycle(c, P, l + 1, P.B[l + 1], rho, s);
-  prolong_add<double>(c->lv[l].node, c->lv[l + 1].node, P.nd, P.nc, e, x, s);
+  apply_t<T>(c, P, V, l, 2, x, b, V.R[l], s);
+  restrict_pt<T>(c->lv[l + 1].node, c->lv[l].node, P.nd, P.nc, V.R[l], V.B[l + 1], s);
+  T* e = vcycle_t<T>(c, P, V, l + 1, V.B[l + 1], s);
+  prolong_add<T>(c->lv[l].node, c->lv[l + 1].node, P.nd, P.nc, e, x, s);
   for (int k = 0; k < c->nsmooth; ++k) {
-    apply(c, P, l, 3, rho, x, b, t, nullptr, s);
+    apply_t<T>(c, P, V, l, 3, x, b, t, s);
     std::swap(x, t);
   }
   return x;
+}
+
+// Preconditioner on the CG residual r (FP64 in, FP64 out).
+double* precondition(maple_ctx* c, Phys& P, const double* r, hipStream_t s) {
+  if (c->prec == MAPLE_PREC_FP64) return vcycle_t<double>(c, P, P.v64, 0, r, s);
+  const long long n0 = (long long)P.nc * P.nd * c->lv[0].Ntot;
+  vec_copy<float, double>(n0, r, P.v32.B[0], s);
+  float* z = vcycle_t<float>(c, P, P.v32, 0, P.v32.B[0], s);
+  vec_copy<double, float>(n0, z, P.z64, s);
+  return P.z64;
 }
 
 int poll_done(maple_ctx* c, Phys& P, int slot, bool* all) {
@@ -315,19 +383,20 @@ int enqueue_poll(maple_ctx* c, Phys& P, int slot, hipStream_t s) {
 
 int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStream_t s) {
   Level& L0 = c->lv[0];
+  const bool flex = c->flexible < 0 ? c->prec != MAPLE_PREC_FP64 : c->flexible != 0;
   const long long n0 = (long long)P.nc * P.nd * L0.Ntot;
   fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, c->partial, s);
   reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_FF, s);
   if (c->warm && P.have_fields) {
-    apply(c, P, 0, 2, rho, P.u, P.f, P.r, nullptr, s);
+    apply_fine64(c, P, 2, rho, P.u, P.f, P.r, nullptr, s);
   } else {
     vec_zero<double>(n0, P.u, s);
     vec_copy<double, double>(n0, P.f, P.r, s);
   }
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, c->partial, s);
   reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
-  double* z = vcycle(c, P, 0, P.r, rho, s);
+  double* z = precondition(c, P, P.r, s);
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, c->partial, s);
   reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RZ_NEW, s);
   pcg_init_scalars(c->ncells, P.nc, rtol, P.atol2, P.sc, P.done_d, s);
@@ -336,7 +405,7 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   bool all = false;
   TRY(poll_done(c, P, 0, &all));
   for (int it = 0; it < maxit && !all; ++it) {
-    apply(c, P, 0, 1, rho, P.p, nullptr, P.q, c->partial, s);
+    apply_fine64(c, P, 1, rho, P.p, nullptr, P.q, c->partial, s);
     reduce_partials(L0.fine.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_PQ, s);
     pcg_alpha(c->ncells, P.nc, P.sc, s);
     pcg_update_ur(L0.node, P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, c->partial, s);
@@ -345,10 +414,14 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
     const int slot = (it + 1) & 1;
     TRY(enqueue_poll(c, P, slot, s));
     if (it + 1 < maxit) {
-      z = vcycle(c, P, 0, P.r, rho, s);
+      if (flex) {
+        vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, c->partial, s);
+        reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RZO, s);
+      }
+      z = precondition(c, P, P.r, s);
       vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, c->partial, s);
       reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RZ_NEW, s);
-      pcg_beta(c->ncells, P.nc, P.sc, s);
+      pcg_beta(c->ncells, P.nc, flex, P.sc, s);
       pcg_update_p(L0.node, P.nd, P.nc, P.sc, z, P.p, s);
     }
     // the device is already working on the next iteration while we look at this one
@@ -400,7 +473,7 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
   *out = nullptr;
   if (ncells <= 0 || !dims_host) return fail(MAPLE_EINVAL, "need at least one cell");
   if (nlevels < 2 || nlevels > 12) return fail(MAPLE_EINVAL, "nlevels must be in [2, 12]");
-  if (precision != MAPLE_PREC_FP64) return fail(MAPLE_EINVAL, "unknown precision");
+  if (precision != MAPLE_PREC_FP64 && precision != MAPLE_PREC_MIXED) return fail(MAPLE_EINVAL, "unknown precision");
   for (int i = 0; i < 3 * ncells; ++i) {
     const int d = dims_host[i];
     if (d <= 0 || d % (1 << (nlevels - 1)) != 0)
@@ -527,6 +600,13 @@ int maple_set_solver(maple_ctx* c, double omega, int nsmooth, int warm_start) {
   return 0;
 }
 
+int maple_set_cg(maple_ctx* c, int flexible) {
+  if (!c) return fail(MAPLE_EINVAL, "null context");
+  if (flexible < -1 || flexible > 1) return fail(MAPLE_EINVAL, "flexible must be -1, 0 or 1");
+  c->flexible = flexible;
+  return 0;
+}
+
 int maple_set_smoother(maple_ctx* c, int kind) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
   if (kind != MAPLE_SMOOTH_JACOBI && kind != MAPLE_SMOOTH_L1_JACOBI) return fail(MAPLE_EINVAL, "unknown smoother");
@@ -632,7 +712,7 @@ int maple_apply_operator(maple_ctx* c, int physics, const double* rho, int ncase
   hipStream_t s = (hipStream_t)stream;
   const int keep = P.nc;
   P.nc = ncase;
-  apply(c, P, 0, 0, rho, u, nullptr, v, nullptr, s);
+  apply_fine64(c, P, 0, rho, u, nullptr, v, nullptr, s);
   P.nc = keep;
   HIPCHK(hipGetLastError());
   return 0;
@@ -669,7 +749,7 @@ int maple_timing_enable(maple_ctx* c, int enable) {
 
 int maple_timing_read(maple_ctx* c, int kind, long long* count, double* total_ms) {
   TRY(check_ctx(c));
-  if (kind < 0 || kind > 1 || !count || !total_ms) return fail(MAPLE_EINVAL, "bad timing query");
+  if (kind < 0 || kind > 2 || !count || !total_ms) return fail(MAPLE_EINVAL, "bad timing query");
   Timer& t = c->timers[kind];
   double ms = 0.0;
   if (t.used > 0) HIPCHK(hipEventSynchronize(t.ev[t.used - 1]));

@@ -37,6 +37,9 @@ __device__ __forceinline__ double w1(int t) { return t == 0 ? 1.0 : ((t == 1 || 
 }  // namespace
 
 // MODE: 0 = K u, 2 = f - K u, 3 = u + omega Dinv (f - K u)
+// blockIdx.y = output component i: a thread owns row i of one node for every
+// load case, so each stencil entry is still read exactly once per application
+// while the coarse grids (32^3 nodes and below) get ND times more waves in flight.
 template <int ND, typename T, int MODE>
 __global__ __launch_bounds__(256) void k_stencil_apply(const LevelCell* __restrict__ cells,
                                                        const int* __restrict__ blk_start, int ncells,
@@ -46,50 +49,43 @@ __global__ __launch_bounds__(256) void k_stencil_apply(const LevelCell* __restri
                                                        T* __restrict__ out) {
   LevelCell c; int ci, x, y, z; long long gid;
   if (!node_of(cells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
-  T acc[MAPLE_MAX_CASES][ND];
+  const int i = blockIdx.y;
+  T acc[MAPLE_MAX_CASES];
 #pragma unroll
-  for (int j = 0; j < MAPLE_MAX_CASES; ++j)
+  for (int j = 0; j < MAPLE_MAX_CASES; ++j) acc[j] = T(0);
+  const int xs[3] = {wrap(x - 1, c.nx), x, wrap(x + 1, c.nx)};
+  const int ys[3] = {wrap(y - 1, c.ny), y, wrap(y + 1, c.ny)};
+  const int zs[3] = {wrap(z - 1, c.nz), z, wrap(z + 1, c.nz)};
 #pragma unroll
-    for (int i = 0; i < ND; ++i) acc[j][i] = T(0);
-  for (int dx = -1; dx <= 1; ++dx) {
-    const int X = wrap(x + dx, c.nx);
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int Y = wrap(y + dy, c.ny);
+  for (int dx = 0; dx < 3; ++dx) {
 #pragma unroll
-      for (int dz = -1; dz <= 1; ++dz) {
-        const int Z = wrap(z + dz, c.nz);
-        const int d = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
-        const long long nb = nid(c, X, Y, Z);
-        T blk[ND][ND];
+    for (int dy = 0; dy < 3; ++dy) {
+      const long long line = c.off + ((long long)xs[dx] * c.ny + ys[dy]) * c.nz;
 #pragma unroll
-        for (int i = 0; i < ND; ++i)
+      for (int dz = 0; dz < 3; ++dz) {
+        const int d = dx * 9 + dy * 3 + dz;
+        const long long nb = line + zs[dz];
+        T srow[ND];
 #pragma unroll
-          for (int e = 0; e < ND; ++e) blk[i][e] = S[(long long)(d * ND * ND + i * ND + e) * Ntot + gid];
+        for (int e = 0; e < ND; ++e) srow[e] = S[(long long)(d * ND * ND + i * ND + e) * Ntot + gid];
 #pragma unroll
         for (int j = 0; j < MAPLE_MAX_CASES; ++j) {
           if (j < ncase) {
-            T ue[ND];
 #pragma unroll
-            for (int e = 0; e < ND; ++e) ue[e] = u[(long long)(j * ND + e) * Ntot + nb];
-#pragma unroll
-            for (int i = 0; i < ND; ++i)
-#pragma unroll
-              for (int e = 0; e < ND; ++e) acc[j][i] = fma(blk[i][e], ue[e], acc[j][i]);
+            for (int e = 0; e < ND; ++e) acc[j] = fma(srow[e], u[(long long)(j * ND + e) * Ntot + nb], acc[j]);
           }
         }
       }
     }
   }
+  const T dv = MODE == 3 ? dinv[(long long)i * Ntot + gid] : T(0);
 #pragma unroll
   for (int j = 0; j < MAPLE_MAX_CASES; ++j) {
     if (j < ncase) {
-#pragma unroll
-      for (int i = 0; i < ND; ++i) {
-        const long long idx = (long long)(j * ND + i) * Ntot + gid;
-        if constexpr (MODE == 0) out[idx] = acc[j][i];
-        else if constexpr (MODE == 2) out[idx] = f[idx] - acc[j][i];
-        else out[idx] = u[idx] + omega * dinv[(long long)i * Ntot + gid] * (f[idx] - acc[j][i]);
-      }
+      const long long idx = (long long)(j * ND + i) * Ntot + gid;
+      if constexpr (MODE == 0) out[idx] = acc[j];
+      else if constexpr (MODE == 2) out[idx] = f[idx] - acc[j];
+      else out[idx] = u[idx] + omega * dv * (f[idx] - acc[j]);
     }
   }
 }
@@ -97,16 +93,16 @@ __global__ __launch_bounds__(256) void k_stencil_apply(const LevelCell* __restri
 template <int ND, typename T>
 void stencil_apply(const Launch& L, int mode, int ncase, const T* S, const T* u, const T* f, const T* dinv,
                    T omega, T* out, double* /*partial*/, hipStream_t s) {
-  if (mode == 0) k_stencil_apply<ND, T, 0><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, S, u, f, dinv, omega, out);
-  else if (mode == 2) k_stencil_apply<ND, T, 2><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, S, u, f, dinv, omega, out);
-  else k_stencil_apply<ND, T, 3><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, S, u, f, dinv, omega, out);
+  dim3 grid(L.nblocks, ND);
+  if (mode == 0) k_stencil_apply<ND, T, 0><<<grid, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, S, u, f, dinv, omega, out);
+  else if (mode == 2) k_stencil_apply<ND, T, 2><<<grid, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, S, u, f, dinv, omega, out);
+  else k_stencil_apply<ND, T, 3><<<grid, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, S, u, f, dinv, omega, out);
 }
 template void stencil_apply<3, double>(const Launch&, int, int, const double*, const double*, const double*, const double*, double, double*, double*, hipStream_t);
 template void stencil_apply<1, double>(const Launch&, int, int, const double*, const double*, const double*, const double*, double, double*, double*, hipStream_t);
 template void stencil_apply<3, float>(const Launch&, int, int, const float*, const float*, const float*, const float*, float, float*, double*, hipStream_t);
 template void stencil_apply<1, float>(const Launch&, int, int, const float*, const float*, const float*, const float*, float, float*, double*, hipStream_t);
 
-// point diagonal, or (l1) the absolute row sum of the assembled operator
 template <typename T>
 __global__ __launch_bounds__(256) void k_stencil_dinv(const LevelCell* __restrict__ cells,
                                                       const int* __restrict__ blk_start, int ncells,
@@ -243,92 +239,72 @@ void stencil_galerkin(const Launch& Lc, const Launch& Lf, int nd, const T* Sf, T
 template void stencil_galerkin<double>(const Launch&, const Launch&, int, const double*, double*, hipStream_t);
 template void stencil_galerkin<float>(const Launch&, const Launch&, int, const float*, float*, hipStream_t);
 
-// b_c(N) = sum_{s in {-1,0,1}^3} w(s) r_f(2N+s)
-template <int ND, typename T>
+// b_c(N) = sum_{s in {-1,0,1}^3} w(s) r_f(2N+s); blockIdx.y = plane (case*ND + component)
+template <typename T>
 __global__ __launch_bounds__(256) void k_restrict(const LevelCell* __restrict__ ccells,
                                                   const int* __restrict__ blk_start, int ncells, long long Ntot_c,
-                                                  const LevelCell* __restrict__ fcells, long long Ntot_f, int ncase,
+                                                  const LevelCell* __restrict__ fcells, long long Ntot_f,
                                                   const T* __restrict__ rf, T* __restrict__ bc) {
   LevelCell c; int ci, x, y, z; long long gid;
   if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
   const LevelCell cf = fcells[ci];
-  T acc[MAPLE_MAX_CASES][ND];
+  const T* __restrict__ r = rf + (long long)blockIdx.y * Ntot_f + cf.off;
+  const int fx[3] = {wrap(2 * x - 1, cf.nx), 2 * x, 2 * x + 1};
+  const int fy[3] = {wrap(2 * y - 1, cf.ny), 2 * y, 2 * y + 1};
+  const int fz[3] = {wrap(2 * z - 1, cf.nz), 2 * z, 2 * z + 1};
+  T acc = T(0);
 #pragma unroll
-  for (int j = 0; j < MAPLE_MAX_CASES; ++j)
+  for (int sx = 0; sx < 3; ++sx)
 #pragma unroll
-    for (int i = 0; i < ND; ++i) acc[j][i] = T(0);
-  for (int sx = -1; sx <= 1; ++sx) {
-    const int fx = wrap(2 * x + sx, cf.nx);
-    for (int sy = -1; sy <= 1; ++sy) {
-      const int fy = wrap(2 * y + sy, cf.ny);
+    for (int sy = 0; sy < 3; ++sy) {
+      const long long line = ((long long)fx[sx] * cf.ny + fy[sy]) * cf.nz;
 #pragma unroll
-      for (int sz = -1; sz <= 1; ++sz) {
-        const int fz = wrap(2 * z + sz, cf.nz);
-        const T w = (T)(w1(sx) * w1(sy) * w1(sz));
-        const long long fn = cf.off + ((long long)fx * cf.ny + fy) * cf.nz + fz;
-#pragma unroll
-        for (int j = 0; j < MAPLE_MAX_CASES; ++j)
-          if (j < ncase)
-#pragma unroll
-            for (int i = 0; i < ND; ++i) acc[j][i] = fma(w, rf[(long long)(j * ND + i) * Ntot_f + fn], acc[j][i]);
+      for (int sz = 0; sz < 3; ++sz) {
+        const T w = (T)((sx == 1 ? 1.0 : 0.5) * (sy == 1 ? 1.0 : 0.5) * (sz == 1 ? 1.0 : 0.5));
+        acc = fma(w, r[line + fz[sz]], acc);
       }
     }
-  }
-#pragma unroll
-  for (int j = 0; j < MAPLE_MAX_CASES; ++j)
-    if (j < ncase)
-#pragma unroll
-      for (int i = 0; i < ND; ++i) bc[(long long)(j * ND + i) * Ntot_c + gid] = acc[j][i];
+  bc[(long long)blockIdx.y * Ntot_c + gid] = acc;
 }
 
 template <typename T>
 void restrict_pt(const Launch& Lc, const Launch& Lf, int nd, int ncase, const T* rf, T* bc, hipStream_t s) {
-  if (nd == 3) k_restrict<3, T><<<Lc.nblocks, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, Lf.Ntot, ncase, rf, bc);
-  else k_restrict<1, T><<<Lc.nblocks, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, Lf.Ntot, ncase, rf, bc);
+  dim3 grid(Lc.nblocks, nd * ncase);
+  k_restrict<T><<<grid, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, Lf.Ntot, rf, bc);
 }
 template void restrict_pt<double>(const Launch&, const Launch&, int, int, const double*, double*, hipStream_t);
 template void restrict_pt<float>(const Launch&, const Launch&, int, int, const float*, float*, hipStream_t);
 
-// u_f(n) += sum over the <= 8 coarse nodes that interpolate to n
-template <int ND, typename T>
+// u_f(n) += sum over the <= 8 coarse nodes that interpolate to n; blockIdx.y = plane
+template <typename T>
 __global__ __launch_bounds__(256) void k_prolong_add(const LevelCell* __restrict__ fcells,
                                                      const int* __restrict__ blk_start, int ncells, long long Ntot_f,
-                                                     const LevelCell* __restrict__ ccells, long long Ntot_c, int ncase,
+                                                     const LevelCell* __restrict__ ccells, long long Ntot_c,
                                                      const T* __restrict__ ec, T* __restrict__ uf) {
   LevelCell c; int ci, x, y, z; long long gid;
   if (!node_of(fcells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
   const LevelCell cc = ccells[ci];
+  const T* __restrict__ e = ec + (long long)blockIdx.y * Ntot_c + cc.off;
   const int bx = x >> 1, by = y >> 1, bz = z >> 1;
   const int rx = x & 1, ry = y & 1, rz = z & 1;
-  T acc[MAPLE_MAX_CASES][ND];
-#pragma unroll
-  for (int j = 0; j < MAPLE_MAX_CASES; ++j)
-#pragma unroll
-    for (int i = 0; i < ND; ++i) acc[j][i] = T(0);
+  const int cx[2] = {bx, rx ? wrap(bx + 1, cc.nx) : bx};
+  const int cy[2] = {by, ry ? wrap(by + 1, cc.ny) : by};
+  const int cz[2] = {bz, rz ? wrap(bz + 1, cc.nz) : bz};
+  const T w = (T)((rx ? 0.5 : 1.0) * (ry ? 0.5 : 1.0) * (rz ? 0.5 : 1.0));
+  T acc = T(0);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int ox = (k >> 2) & 1, oy = (k >> 1) & 1, oz = k & 1;
     if ((ox && !rx) || (oy && !ry) || (oz && !rz)) continue;
-    const T w = (T)((rx ? 0.5 : 1.0) * (ry ? 0.5 : 1.0) * (rz ? 0.5 : 1.0));
-    const long long cn = cc.off + ((long long)wrap(bx + ox, cc.nx) * cc.ny + wrap(by + oy, cc.ny)) * cc.nz +
-                         wrap(bz + oz, cc.nz);
-#pragma unroll
-    for (int j = 0; j < MAPLE_MAX_CASES; ++j)
-      if (j < ncase)
-#pragma unroll
-        for (int i = 0; i < ND; ++i) acc[j][i] = fma(w, ec[(long long)(j * ND + i) * Ntot_c + cn], acc[j][i]);
+    acc = fma(w, e[((long long)cx[ox] * cc.ny + cy[oy]) * cc.nz + cz[oz]], acc);
   }
-#pragma unroll
-  for (int j = 0; j < MAPLE_MAX_CASES; ++j)
-    if (j < ncase)
-#pragma unroll
-      for (int i = 0; i < ND; ++i) uf[(long long)(j * ND + i) * Ntot_f + gid] += acc[j][i];
+  uf[(long long)blockIdx.y * Ntot_f + gid] += acc;
 }
 
 template <typename T>
 void prolong_add(const Launch& Lf, const Launch& Lc, int nd, int ncase, const T* ec, T* uf, hipStream_t s) {
-  if (nd == 3) k_prolong_add<3, T><<<Lf.nblocks, 256, 0, s>>>(Lf.cells, Lf.blk_start, Lf.ncells, Lf.Ntot, Lc.cells, Lc.Ntot, ncase, ec, uf);
-  else k_prolong_add<1, T><<<Lf.nblocks, 256, 0, s>>>(Lf.cells, Lf.blk_start, Lf.ncells, Lf.Ntot, Lc.cells, Lc.Ntot, ncase, ec, uf);
+  dim3 grid(Lf.nblocks, nd * ncase);
+  k_prolong_add<T><<<grid, 256, 0, s>>>(Lf.cells, Lf.blk_start, Lf.ncells, Lf.Ntot, Lc.cells, Lc.Ntot, ec, uf);
 }
 template void prolong_add<double>(const Launch&, const Launch&, int, int, const double*, double*, hipStream_t);
 template void prolong_add<float>(const Launch&, const Launch&, int, int, const float*, float*, hipStream_t);

@@ -68,6 +68,7 @@ struct CaseScalars {   // one per (cell, case)
   double ff;           // ||f||^2
   double tgt2;         // squared residual target
   double rz, rz_new, pq, rr;
+  double rzo;          // (r_new, z_old): the flexible (Polak-Ribiere) correction of beta
   double alpha, beta;
   int done, iters;
   int pad_[2];
@@ -75,7 +76,7 @@ struct CaseScalars {   // one per (cell, case)
 // sum partial[blk*6 + j] over each cell's blocks into dst (a member of CaseScalars)
 void reduce_partials(const int* blk_start, int ncells, int ncase, const double* partial,
                      CaseScalars* sc, int which, hipStream_t s);
-enum { SC_FF = 0, SC_RZ_NEW, SC_PQ, SC_RR };
+enum { SC_FF = 0, SC_RZ_NEW, SC_PQ, SC_RR, SC_RZO };
 // elementwise (cell-aligned 256-node blocks):
 template <typename T>
 void vec_scale_dinv(const Launch& L, int nd, int ncase, T omega, const T* dinv, const T* b, T* x, hipStream_t s);
@@ -92,7 +93,8 @@ void pcg_init_scalars(int ncells, int ncase, double rtol, const double* atol2, C
                       hipStream_t s);
 void pcg_alpha(int ncells, int ncase, CaseScalars* sc, hipStream_t s);
 void pcg_check(int ncells, int ncase, CaseScalars* sc, int* done_flags, hipStream_t s);
-void pcg_beta(int ncells, int ncase, CaseScalars* sc, hipStream_t s);
+// flexible: beta = (rz_new - rzo) / rz (valid for a variable or slightly non-symmetric preconditioner)
+void pcg_beta(int ncells, int ncase, bool flexible, CaseScalars* sc, hipStream_t s);
 template <typename Td, typename Ts>
 void vec_copy(long long n, const Ts* src, Td* dst, hipStream_t s);
 template <typename T>

@@ -20,6 +20,7 @@ __global__ __launch_bounds__(64) void k_reduce_partials(const int* __restrict__ 
     if (which == SC_FF) s.ff = v;
     else if (which == SC_RZ_NEW) s.rz_new = v;
     else if (which == SC_PQ) s.pq = v;
+    else if (which == SC_RZO) s.rzo = v;
     else s.rr = v;
   }
 }
@@ -177,17 +178,18 @@ void pcg_check(int ncells, int ncase, CaseScalars* sc, int* done_flags, hipStrea
   k_pcg_check<<<(n + 63) / 64, 64, 0, s>>>(n, sc, done_flags);
 }
 
-__global__ void k_pcg_beta(int n, CaseScalars* __restrict__ sc) {
+__global__ void k_pcg_beta(int n, int flexible, CaseScalars* __restrict__ sc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   CaseScalars& s = sc[t];
-  s.beta = (!s.done && s.rz != 0.0) ? s.rz_new / s.rz : 0.0;
+  const double num = flexible ? s.rz_new - s.rzo : s.rz_new;
+  s.beta = (!s.done && s.rz != 0.0) ? num / s.rz : 0.0;
   s.rz = s.rz_new;
 }
 
-void pcg_beta(int ncells, int ncase, CaseScalars* sc, hipStream_t s) {
+void pcg_beta(int ncells, int ncase, bool flexible, CaseScalars* sc, hipStream_t s) {
   const int n = ncells * ncase;
-  k_pcg_beta<<<(n + 63) / 64, 64, 0, s>>>(n, sc);
+  k_pcg_beta<<<(n + 63) / 64, 64, 0, s>>>(n, flexible ? 1 : 0, sc);
 }
 
 template <typename Td, typename Ts>
