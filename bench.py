@@ -92,6 +92,40 @@ def run_gpu(steps: int, warmup: int, profile_kernel: bool = True, warm_start: bo
     return out
 
 
+def run_batched(steps: int, warmup: int, precision: str = "fp64", maxit: int = 5000):
+    """The 64-cell mixed-resolution batch: every OC step advances all cells with one set of launches."""
+    import numpy as np
+    import torch
+
+    import maple_amd
+
+    dev = torch.device("cuda", 0)
+    spec = maple_amd.batched_workload()
+    cells = maple_amd.CellBatch([c["r"] for c in spec], nlevels=NLEVELS, device=dev, precision=precision)
+    drv = maple_amd.OCDriver(cells, [c["weights"] for c in spec], [c["volfrac"] for c in spec], move=0.2, xmin=1e-3,
+                             filter_k=3, rtol=1e-6, maxit=maxit)
+    x0 = torch.from_numpy(np.concatenate([c["x0"] for c in spec])).to(dev)
+    x = x0.clone()
+    for _ in range(warmup):
+        x, info = drv.step(x)
+    torch.cuda.synchronize()
+    x = x0.clone()
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        x, info = drv.step(x)
+        iters.append(info["iters"].max(axis=0).tolist())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    elems = sum(c["r"] ** 3 for c in spec)
+    it = np.array(iters)
+    return {"steps_per_s": steps / dt, "ms_per_step": dt * 1e3 / steps, "cells": len(spec), "elements": elems,
+            "cell_oc_iters_per_s": len(spec) * steps / dt,
+            "equiv_64cube_oc_iters_per_s": elems / R ** 3 * steps / dt,
+            "elastic_cg_mean_of_step_max": float(it[:, :6].max(axis=1).mean()),
+            "thermal_cg_mean_of_step_max": float(it[:, 6:].max(axis=1).mean())}
+
+
 def cpu_baseline(n_iter: int):
     import numpy as np
 
@@ -116,6 +150,8 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--forms", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--batched", default="", help="also run the 64-cell batch in these precisions (fp64,mixed)")
+    ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--warm-start", action="store_true", help="start each CG from the previous OC iterate's fields")
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -169,6 +205,8 @@ def main():
                             "algorithmic_flop_per_launch": k["flop_per_launch"],
                             "roof": "vector FMA and MFMA share the MI355X peak for this precision "
                                     "(FP64 78.6, FP32 157.3 TFLOP/s)"}
+    for p in [q.strip() for q in args.batched.split(",") if q.strip()]:
+        line.setdefault("batched", {})[p] = run_batched(args.batched_steps, 1, precision=p)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_iters)
     print(json.dumps(line), flush=True)

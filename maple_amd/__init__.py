@@ -15,11 +15,12 @@ from .model import IHD, make_cfg
 from .objective import (objective_from_spec, objective_requires, parse_objective_spec, shear_objective,
                         stiffness_objective, thermal_objective, bulk_objective)
 from .optim import (OCDriver, assemble_F_fast, assemble_F_fast_scalar, avg_filter3d_x_periodic,
-                    build_unitcell_batch, oc_update, x0_sphere_void, x0_uniform_noise)
+                    build_unitcell_batch, oc_update, x0_sphere_void, x0_uniform_noise,
+                    batched_workload, simplex_grid)
 
 __all__ = [
     "CellBatch", "hex8_matrices", "max_levels", "IHD", "make_cfg", "objective_from_spec", "objective_requires",
     "parse_objective_spec", "shear_objective", "stiffness_objective", "thermal_objective", "bulk_objective",
     "OCDriver", "assemble_F_fast", "assemble_F_fast_scalar", "avg_filter3d_x_periodic", "build_unitcell_batch",
-    "oc_update", "x0_sphere_void", "x0_uniform_noise",
+    "oc_update", "x0_sphere_void", "x0_uniform_noise", "batched_workload", "simplex_grid",
 ]

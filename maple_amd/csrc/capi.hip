@@ -77,6 +77,8 @@ struct Phys {
   std::vector<CaseScalars> hsc;
   int* done_d = nullptr;
   int* done_h = nullptr;   // pinned, 2 slots
+  std::vector<Launch> ln;  // per-level node launches that skip converged (cell, case) pairs
+  Launch lf;               // level-0 operator launch, same
   double* atol2 = nullptr;
   bool have_fields = false, have_energy = false;
 };
@@ -241,6 +243,15 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
   TRY(dalloc(c, &P.Se, (size_t)nc * (nc + 1) / 2 * c->lv[0].Ntot));
   TRY(dalloc(c, &P.sc, (size_t)c->ncells * nc));
   TRY(dalloc(c, &P.done_d, (size_t)c->ncells * nc));
+  P.ln.resize(L);
+  for (int l = 0; l < L; ++l) {
+    P.ln[l] = c->lv[l].node;
+    P.ln[l].done = P.done_d;
+    P.ln[l].ndc = nc;
+  }
+  P.lf = c->lv[0].fine;
+  P.lf.done = P.done_d;
+  P.lf.ndc = nc;
   P.hsc.resize((size_t)c->ncells * nc);
   void* hp = nullptr;
   HIPCHK(hipHostMalloc(&hp, sizeof(int) * 2 * c->ncells * nc, hipHostMallocDefault));
@@ -276,12 +287,13 @@ void timed_end(maple_ctx* c, int kind, hipStream_t s) {
 void apply_fine64(maple_ctx* c, Phys& P, int mode, const double* rho, const double* u, const double* f, double* out,
                   double* partial, hipStream_t s) {
   const int kind = P.nd == 3 ? 0 : 1;
+  const Launch& L = mode == 0 ? c->lv[0].fine : P.lf;
   timed_begin(c, kind, s);
   if (P.nd == 3)
-    fine_apply<3, double>(c->lv[0].fine, mode, P.nc, P.Ke, rho, u, f, P.v64.dinv.empty() ? nullptr : P.v64.dinv[0],
+    fine_apply<3, double>(L, mode, P.nc, P.Ke, rho, u, f, P.v64.dinv.empty() ? nullptr : P.v64.dinv[0],
                           c->omega, out, partial, s);
   else
-    fine_apply<1, double>(c->lv[0].fine, mode, P.nc, P.Ke, rho, u, f, P.v64.dinv.empty() ? nullptr : P.v64.dinv[0],
+    fine_apply<1, double>(L, mode, P.nc, P.Ke, rho, u, f, P.v64.dinv.empty() ? nullptr : P.v64.dinv[0],
                           c->omega, out, partial, s);
   timed_end(c, kind, s);
 }
@@ -293,15 +305,15 @@ void apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const T* u, const
     const int kind = P.nd == 3 ? (sizeof(T) == 8 ? 0 : 2) : (sizeof(T) == 8 ? 1 : -1);
     if (kind >= 0) timed_begin(c, kind, s);
     if (P.nd == 3)
-      fine_apply<3, T>(c->lv[0].fine, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+      fine_apply<3, T>(P.lf, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
     else
-      fine_apply<1, T>(c->lv[0].fine, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+      fine_apply<1, T>(P.lf, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
     if (kind >= 0) timed_end(c, kind, s);
   } else {
     if (P.nd == 3)
-      stencil_apply<3, T>(c->lv[l].node, mode, P.nc, V.S[l], u, f, V.dinv[l], (T)c->omega, out, nullptr, s);
+      stencil_apply<3, T>(P.ln[l], mode, P.nc, V.S[l], u, f, V.dinv[l], (T)c->omega, out, nullptr, s);
     else
-      stencil_apply<1, T>(c->lv[l].node, mode, P.nc, V.S[l], u, f, V.dinv[l], (T)c->omega, out, nullptr, s);
+      stencil_apply<1, T>(P.ln[l], mode, P.nc, V.S[l], u, f, V.dinv[l], (T)c->omega, out, nullptr, s);
   }
 }
 
@@ -334,20 +346,20 @@ template <typename T>
 T* vcycle_t(maple_ctx* c, Phys& P, VC<T>& V, int l, const T* b, hipStream_t s) {
   const int L = c->nlevels;
   if (l == L - 1) {
-    dense_apply<T>(c->lv[l].node, P.ddc, P.maxn, P.nd, P.nc, P.A, b, V.X[l], s);
+    dense_apply<T>(P.ln[l], P.ddc, P.maxn, P.nd, P.nc, P.A, b, V.X[l], s);
     return V.X[l];
   }
   T* x = V.X[l];
   T* t = V.W[l];
-  vec_scale_dinv<T>(c->lv[l].node, P.nd, P.nc, (T)c->omega, V.dinv[l], b, x, s);
+  vec_scale_dinv<T>(P.ln[l], P.nd, P.nc, (T)c->omega, V.dinv[l], b, x, s);
   for (int k = 1; k < c->nsmooth; ++k) {
     apply_t<T>(c, P, V, l, 3, x, b, t, s);
     std::swap(x, t);
   }
   apply_t<T>(c, P, V, l, 2, x, b, V.R[l], s);
-  restrict_pt<T>(c->lv[l + 1].node, c->lv[l].node, P.nd, P.nc, V.R[l], V.B[l + 1], s);
+  restrict_pt<T>(P.ln[l + 1], P.ln[l], P.nd, P.nc, V.R[l], V.B[l + 1], s);
   T* e = vcycle_t<T>(c, P, V, l + 1, V.B[l + 1], s);
-  prolong_add<T>(c->lv[l].node, c->lv[l + 1].node, P.nd, P.nc, e, x, s);
+  prolong_add<T>(P.ln[l], P.ln[l + 1], P.nd, P.nc, e, x, s);
   for (int k = 0; k < c->nsmooth; ++k) {
     apply_t<T>(c, P, V, l, 3, x, b, t, s);
     std::swap(x, t);
@@ -385,6 +397,7 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   Level& L0 = c->lv[0];
   const bool flex = c->flexible < 0 ? c->prec != MAPLE_PREC_FP64 : c->flexible != 0;
   const long long n0 = (long long)P.nc * P.nd * L0.Ntot;
+  vec_zero<int>((long long)c->ncells * P.nc, P.done_d, s);   // no case is converged before the start
   fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, c->partial, s);
   reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_FF, s);

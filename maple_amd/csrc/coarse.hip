@@ -43,60 +43,65 @@ __device__ __forceinline__ double w1(int t) { return t == 0 ? 1.0 : ((t == 1 || 
 template <int ND, typename T, int MODE>
 __global__ __launch_bounds__(256) void k_stencil_apply(const LevelCell* __restrict__ cells,
                                                        const int* __restrict__ blk_start, int ncells,
-                                                       long long Ntot, int ncase, const T* __restrict__ S,
+                                                       long long Ntot, const T* __restrict__ S,
                                                        const T* __restrict__ u, const T* __restrict__ f,
                                                        const T* __restrict__ dinv, T omega,
-                                                       T* __restrict__ out) {
+                                                       T* __restrict__ out, const int* __restrict__ done, int ndc) {
   LevelCell c; int ci, x, y, z; long long gid;
   if (!node_of(cells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
+  if (cell_done(done, ndc, ci)) return;
+  // the load-case count is a compile-time constant (6 elastic, 3 thermal) so that
+  // every load of a neighbour's planes can be issued without a branch
+  constexpr int NC = ND == 3 ? 6 : 3;
   const int i = blockIdx.y;
-  T acc[MAPLE_MAX_CASES];
+  T acc[NC];
 #pragma unroll
-  for (int j = 0; j < MAPLE_MAX_CASES; ++j) acc[j] = T(0);
+  for (int j = 0; j < NC; ++j) acc[j] = T(0);
   const int xs[3] = {wrap(x - 1, c.nx), x, wrap(x + 1, c.nx)};
   const int ys[3] = {wrap(y - 1, c.ny), y, wrap(y + 1, c.ny)};
   const int zs[3] = {wrap(z - 1, c.nz), z, wrap(z + 1, c.nz)};
+  // uniform plane bases (scalar registers) + 32-bit node offsets per lane
+  const T* up[NC * ND];
+#pragma unroll
+  for (int q = 0; q < NC * ND; ++q) up[q] = u + (long long)q * Ntot + c.off;
+  const unsigned g32 = (unsigned)(gid - c.off);
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx) {
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy) {
-      const long long line = c.off + ((long long)xs[dx] * c.ny + ys[dy]) * c.nz;
+      const unsigned line = (unsigned)((xs[dx] * c.ny + ys[dy]) * c.nz);
 #pragma unroll
       for (int dz = 0; dz < 3; ++dz) {
         const int d = dx * 9 + dy * 3 + dz;
-        const long long nb = line + zs[dz];
+        const unsigned nb = line + (unsigned)zs[dz];
         T srow[ND];
 #pragma unroll
-        for (int e = 0; e < ND; ++e) srow[e] = S[(long long)(d * ND * ND + i * ND + e) * Ntot + gid];
+        for (int e = 0; e < ND; ++e) srow[e] = (S + (long long)(d * ND * ND + i * ND + e) * Ntot + c.off)[g32];
 #pragma unroll
-        for (int j = 0; j < MAPLE_MAX_CASES; ++j) {
-          if (j < ncase) {
+        for (int j = 0; j < NC; ++j)
 #pragma unroll
-            for (int e = 0; e < ND; ++e) acc[j] = fma(srow[e], u[(long long)(j * ND + e) * Ntot + nb], acc[j]);
-          }
-        }
+          for (int e = 0; e < ND; ++e) acc[j] = fma(srow[e], up[j * ND + e][nb], acc[j]);
       }
     }
   }
   const T dv = MODE == 3 ? dinv[(long long)i * Ntot + gid] : T(0);
 #pragma unroll
-  for (int j = 0; j < MAPLE_MAX_CASES; ++j) {
-    if (j < ncase) {
-      const long long idx = (long long)(j * ND + i) * Ntot + gid;
-      if constexpr (MODE == 0) out[idx] = acc[j];
-      else if constexpr (MODE == 2) out[idx] = f[idx] - acc[j];
-      else out[idx] = u[idx] + omega * dv * (f[idx] - acc[j]);
-    }
+  for (int j = 0; j < NC; ++j) {
+    const long long idx = (long long)(j * ND + i) * Ntot + gid;
+    if constexpr (MODE == 0) out[idx] = acc[j];
+    else if constexpr (MODE == 2) out[idx] = f[idx] - acc[j];
+    else out[idx] = u[idx] + omega * dv * (f[idx] - acc[j]);
   }
 }
 
 template <int ND, typename T>
 void stencil_apply(const Launch& L, int mode, int ncase, const T* S, const T* u, const T* f, const T* dinv,
                    T omega, T* out, double* /*partial*/, hipStream_t s) {
+  (void)ncase;   // always all load cases of the physics (6 elastic, 3 thermal)
   dim3 grid(L.nblocks, ND);
-  if (mode == 0) k_stencil_apply<ND, T, 0><<<grid, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, S, u, f, dinv, omega, out);
-  else if (mode == 2) k_stencil_apply<ND, T, 2><<<grid, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, S, u, f, dinv, omega, out);
-  else k_stencil_apply<ND, T, 3><<<grid, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, S, u, f, dinv, omega, out);
+  if (mode == 0) k_stencil_apply<ND, T, 0><<<grid, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, S, u, f, dinv, omega, out, L.done, L.ndc);
+  else if (mode == 2) k_stencil_apply<ND, T, 2><<<grid, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, S, u, f, dinv, omega, out, L.done, L.ndc);
+  else k_stencil_apply<ND, T, 3><<<grid, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, S, u, f, dinv, omega, out, L.done, L.ndc);
 }
 template void stencil_apply<3, double>(const Launch&, int, int, const double*, const double*, const double*, const double*, double, double*, double*, hipStream_t);
 template void stencil_apply<1, double>(const Launch&, int, int, const double*, const double*, const double*, const double*, double, double*, double*, hipStream_t);
@@ -244,9 +249,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_restrict(const LevelCell* __restrict__ ccells,
                                                   const int* __restrict__ blk_start, int ncells, long long Ntot_c,
                                                   const LevelCell* __restrict__ fcells, long long Ntot_f,
-                                                  const T* __restrict__ rf, T* __restrict__ bc) {
+                                                  const T* __restrict__ rf, T* __restrict__ bc,
+                                                  const int* __restrict__ done, int ndc, int nd) {
   LevelCell c; int ci, x, y, z; long long gid;
   if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
+  if (case_done(done, ndc, ci, blockIdx.y / nd)) return;
   const LevelCell cf = fcells[ci];
   const T* __restrict__ r = rf + (long long)blockIdx.y * Ntot_f + cf.off;
   const int fx[3] = {wrap(2 * x - 1, cf.nx), 2 * x, 2 * x + 1};
@@ -270,7 +277,7 @@ __global__ __launch_bounds__(256) void k_restrict(const LevelCell* __restrict__ 
 template <typename T>
 void restrict_pt(const Launch& Lc, const Launch& Lf, int nd, int ncase, const T* rf, T* bc, hipStream_t s) {
   dim3 grid(Lc.nblocks, nd * ncase);
-  k_restrict<T><<<grid, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, Lf.Ntot, rf, bc);
+  k_restrict<T><<<grid, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, Lf.Ntot, rf, bc, Lc.done, Lc.ndc, nd);
 }
 template void restrict_pt<double>(const Launch&, const Launch&, int, int, const double*, double*, hipStream_t);
 template void restrict_pt<float>(const Launch&, const Launch&, int, int, const float*, float*, hipStream_t);
@@ -280,9 +287,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_prolong_add(const LevelCell* __restrict__ fcells,
                                                      const int* __restrict__ blk_start, int ncells, long long Ntot_f,
                                                      const LevelCell* __restrict__ ccells, long long Ntot_c,
-                                                     const T* __restrict__ ec, T* __restrict__ uf) {
+                                                     const T* __restrict__ ec, T* __restrict__ uf,
+                                                     const int* __restrict__ done, int ndc, int nd) {
   LevelCell c; int ci, x, y, z; long long gid;
   if (!node_of(fcells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
+  if (case_done(done, ndc, ci, blockIdx.y / nd)) return;
   const LevelCell cc = ccells[ci];
   const T* __restrict__ e = ec + (long long)blockIdx.y * Ntot_c + cc.off;
   const int bx = x >> 1, by = y >> 1, bz = z >> 1;
@@ -304,7 +313,7 @@ __global__ __launch_bounds__(256) void k_prolong_add(const LevelCell* __restrict
 template <typename T>
 void prolong_add(const Launch& Lf, const Launch& Lc, int nd, int ncase, const T* ec, T* uf, hipStream_t s) {
   dim3 grid(Lf.nblocks, nd * ncase);
-  k_prolong_add<T><<<grid, 256, 0, s>>>(Lf.cells, Lf.blk_start, Lf.ncells, Lf.Ntot, Lc.cells, Lc.Ntot, ec, uf);
+  k_prolong_add<T><<<grid, 256, 0, s>>>(Lf.cells, Lf.blk_start, Lf.ncells, Lf.Ntot, Lc.cells, Lc.Ntot, ec, uf, Lf.done, Lf.ndc, nd);
 }
 template void prolong_add<double>(const Launch&, const Launch&, int, int, const double*, double*, hipStream_t);
 template void prolong_add<float>(const Launch&, const Launch&, int, int, const float*, float*, hipStream_t);

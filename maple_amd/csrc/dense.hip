@@ -264,11 +264,13 @@ void dense_inverse(const DenseCell* dcells, int ncells, int max_nt, double* A, d
 template <int ND, typename T>
 __global__ __launch_bounds__(256) void k_dense_apply(const DenseCell* __restrict__ dcells, long long Ntot,
                                                      int ncase, const double* __restrict__ Ainv,
-                                                     const T* __restrict__ b, T* __restrict__ x) {
+                                                     const T* __restrict__ b, T* __restrict__ x,
+                                                     const int* __restrict__ done) {
   const int ci = blockIdx.y;
   const DenseCell dc = dcells[ci];
   const int row = blockIdx.x;
   if (row >= dc.n) return;
+  if (cell_done(done, ncase, ci)) return;
   double acc[MAPLE_MAX_CASES];
 #pragma unroll
   for (int j = 0; j < MAPLE_MAX_CASES; ++j) acc[j] = 0.0;
@@ -290,8 +292,8 @@ template <typename T>
 void dense_apply(const Launch& Lc, const DenseCell* dcells, int max_n, int nd, int ncase, const double* Ainv,
                  const T* b, T* x, hipStream_t s) {
   dim3 grid(max_n, Lc.ncells);
-  if (nd == 3) k_dense_apply<3, T><<<grid, 256, 0, s>>>(dcells, Lc.Ntot, ncase, Ainv, b, x);
-  else k_dense_apply<1, T><<<grid, 256, 0, s>>>(dcells, Lc.Ntot, ncase, Ainv, b, x);
+  if (nd == 3) k_dense_apply<3, T><<<grid, 256, 0, s>>>(dcells, Lc.Ntot, ncase, Ainv, b, x, Lc.done);
+  else k_dense_apply<1, T><<<grid, 256, 0, s>>>(dcells, Lc.Ntot, ncase, Ainv, b, x, Lc.done);
 }
 template void dense_apply<double>(const Launch&, const DenseCell*, int, int, int, const double*, const double*, double*, hipStream_t);
 template void dense_apply<float>(const Launch&, const DenseCell*, int, int, int, const double*, const float*, float*, hipStream_t);

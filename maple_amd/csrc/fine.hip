@@ -73,9 +73,10 @@ __global__ __launch_bounds__(256) void k_fine_apply(
     const LevelCell* __restrict__ cells, const int* __restrict__ blk_start, int ncells,
     long long Ntot, int ncase, const T* __restrict__ Ke, const T* __restrict__ rho,
     const T* __restrict__ u, const T* __restrict__ f, const T* __restrict__ dinv, T omega,
-    T* __restrict__ out, double* __restrict__ partial) {
+    T* __restrict__ out, double* __restrict__ partial, const int* __restrict__ done, int ndc) {
   constexpr int NE = 8 * ND;
   const int ci = find_cell(blk_start, ncells, lbid());
+  if (case_done(done, ndc, ci, blockIdx.y)) return;   // converged: nothing of this case is read again
   const LevelCell c = cells[ci];
   const int wave = (lbid() - blk_start[ci]) * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(256) void k_fine_apply(
       for (int mm = 0; mm < 8; ++mm)
 #pragma unroll
         for (int e = 0; e < ND; ++e)
-          ue[ND * mm + e] = uj[(long long)e * Ntot + lb[a + hex_dx(mm)][b + hex_dy(mm)] + zs[hex_dz(mm)]];
+          ue[ND * mm + e] = uj[(long long)e * Ntot + (lb[a + hex_dx(mm)][b + hex_dy(mm)] + zs[hex_dz(mm)])];
       if (a == 1 && b == 1) {
 #pragma unroll
         for (int e = 0; e < ND; ++e) uown[e] = ue[e];   // corner 0 of element (x,y,ze) is node (x,y,ze)
@@ -186,10 +187,10 @@ void fine_apply(const Launch& L, int mode, int ncase, const T* Ke, const T* rho,
                 const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s) {
   dim3 grid(L.nblocks, ncase), block(256);
   switch (mode) {
-    case 0: k_fine_apply<ND, T, 0><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial); break;
-    case 1: k_fine_apply<ND, T, 1><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial); break;
-    case 2: k_fine_apply<ND, T, 2><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial); break;
-    default: k_fine_apply<ND, T, 3><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial); break;
+    case 0: k_fine_apply<ND, T, 0><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
+    case 1: k_fine_apply<ND, T, 1><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
+    case 2: k_fine_apply<ND, T, 2><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
+    default: k_fine_apply<ND, T, 3><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
   }
 }
 

@@ -11,7 +11,22 @@ struct Launch {
   int ncells = 0;
   int nblocks = 0;
   long long Ntot = 0;
+  // optional convergence flags done[cell * ndc + case]: blocks of converged
+  // (cell, case) pairs return at once, so a batch pays only for the cases
+  // still iterating
+  const int* done = nullptr;
+  int ndc = 0;
 };
+
+__device__ __forceinline__ bool case_done(const int* done, int ndc, int ci, int j) {
+  return done != nullptr && done[ci * ndc + j] != 0;
+}
+__device__ __forceinline__ bool cell_done(const int* done, int ndc, int ci) {
+  if (done == nullptr) return false;
+  for (int j = 0; j < ndc; ++j)
+    if (!done[ci * ndc + j]) return false;
+  return true;
+}
 
 // ---- level 0 (fine.hip) ---------------------------------------------------
 int fine_waves(const LevelCell& c);  // wavefronts per cell of k_fine_apply (4 per block)

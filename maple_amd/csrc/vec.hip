@@ -45,6 +45,7 @@ __global__ __launch_bounds__(256) void k_scale_dinv(Launch L, int nd, int ncase,
                                                     T* __restrict__ x) {
   int ci; long long gid;
   if (!vnode(L, ci, gid)) return;
+  if (cell_done(L.done, L.ndc, ci)) return;
   for (int i = 0; i < nd; ++i) {
     const T w = omega * dinv[(long long)i * L.Ntot + gid];
     for (int j = 0; j < ncase; ++j) {
@@ -93,8 +94,8 @@ __global__ __launch_bounds__(256) void k_update_ur(Launch L, int nd, int ncase, 
   const bool ok = vnode(L, ci, gid);
   for (int j = 0; j < ncase; ++j) {
     double v[1] = {0.0};
-    if (ok) {
-      const double al = sc[ci * ncase + j].alpha;
+    const double al = ok ? sc[ci * ncase + j].alpha : 0.0;
+    if (ok && al != 0.0) {   // a converged case keeps u and r untouched (its p may be stale)
       for (int i = 0; i < nd; ++i) {
         const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
         u[idx] = fma(al, p[idx], u[idx]);
@@ -117,6 +118,7 @@ __global__ __launch_bounds__(256) void k_update_p(Launch L, int nd, int ncase, c
   int ci; long long gid;
   if (!vnode(L, ci, gid)) return;
   for (int j = 0; j < ncase; ++j) {
+    if (sc[ci * ncase + j].done) continue;   // a converged case's p is never used again
     const double be = sc[ci * ncase + j].beta;
     for (int i = 0; i < nd; ++i) {
       const long long idx = (long long)(j * nd + i) * L.Ntot + gid;

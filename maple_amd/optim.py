@@ -138,6 +138,32 @@ def x0_uniform_noise(r: int, seed: int = 0, vf=0.25, noise=0.05) -> np.ndarray:
     return np.clip(x, 0.001, 1.0)
 
 
+def simplex_grid(step: float = 0.05):
+    """The (w_E, w_G, w_kappa) grid on the simplex with the given step (231 points at 5 %)."""
+    n = int(round(1.0 / step))
+    return [(i / n, j / n, (n - i - j) / n) for i in range(n + 1) for j in range(n + 1 - i)]
+
+
+def batched_workload(sizes=((32, 16), (48, 32), (64, 16)), base_seed: int = 0):
+    """The mixed-resolution batch: per cell its resolution, seed (base + index), objective
+    weights from the 5 % simplex grid, volume fraction U[0.05, 0.40] and x0 = vf + U(-0.05, 0.05)
+    clipped to [0.001, 1] (numpy default_rng(seed))."""
+    grid = simplex_grid(0.05)
+    cells = []
+    idx = 0
+    for r, count in sizes:
+        for _ in range(count):
+            seed = base_seed + idx
+            rng = np.random.default_rng(seed)
+            wE, wG, wK = grid[int(rng.integers(len(grid)))]
+            vf = float(rng.uniform(0.05, 0.40))
+            x0 = np.clip(vf + rng.uniform(-0.05, 0.05, r ** 3), 0.001, 1.0)
+            w = {k: v for k, v in (("stiffness", wE), ("shear", wG), ("thermal", wK)) if v > 0}
+            cells.append({"r": r, "seed": seed, "weights": w, "volfrac": vf, "x0": x0})
+            idx += 1
+    return cells
+
+
 # ---- the OC loop -------------------------------------------------------------
 
 class OCDriver:
