@@ -89,6 +89,26 @@ int maple_set_smoother(maple_ctx* ctx, int kind);
  * product per iteration), -1 = flexible in MAPLE_PREC_MIXED only (default). */
 int maple_set_cg(maple_ctx* ctx, int flexible);
 
+/* MAPLE_PREC_MIXED only: a solve that has not converged after `iters` CG
+ * iterations builds the FP64 hierarchy of the same density and finishes on the
+ * FP64 V-cycle (0 = never; default 100).  An FP32 V-cycle cannot resolve the
+ * soft modes of solid islands held by 1e-9 voids, where it stagnates (no
+ * counterpart in the reference, whose solver runs in FP64 only). */
+int maple_set_mixed_fallback(maple_ctx* ctx, int iters);
+
+/* Rebuild the coarsest-level dense inverse every `every` homogenizations (1 =
+ * always, the default).  In between, the previous density's SPD inverse serves
+ * as the coarsest solve of the V-cycle: the preconditioner stays SPD and CG
+ * still stops at the requested tolerance (no counterpart in the reference,
+ * which rebuilds its whole hierarchy on every solve, EBE_GMG.py:790-849). */
+int maple_set_coarse_refresh(maple_ctx* ctx, int every);
+
+/* Kernel variants for the tuner (maple_amd/tune.py times them and keeps the
+ * fastest): MAPLE_KERNEL_FINE_OPERATOR, variant 0 (all neighbour values held
+ * in registers) or 1 (loads staged element by element, fewer live registers). */
+#define MAPLE_KERNEL_FINE_OPERATOR 0
+int maple_set_variant(maple_ctx* ctx, int kernel, int variant);
+
 /* Homogenize: for every cell, solve the 6 elastic unit-strain (and/or 3 thermal
  * unit-gradient) periodic cell problems K(rho) u = F(rho) by multigrid-
  * preconditioned CG to ||r|| <= max(rtol ||f||, 1e-14 ||Fe||_F sqrt(E)),

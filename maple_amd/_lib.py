@@ -24,6 +24,7 @@ MAPLE_PREC_FP64 = 0
 MAPLE_PREC_MIXED = 1
 MAPLE_SMOOTH_JACOBI = 0
 MAPLE_SMOOTH_L1_JACOBI = 1
+MAPLE_KERNEL_FINE_OPERATOR = 0
 
 _P = C.c_void_p
 _D = C.c_double
@@ -43,6 +44,9 @@ SIGNATURES = {
     "maple_set_solver": (_I, [_P, _D, _I, _I]),
     "maple_set_smoother": (_I, [_P, _I]),
     "maple_set_cg": (_I, [_P, _I]),
+    "maple_set_mixed_fallback": (_I, [_P, _I]),
+    "maple_set_coarse_refresh": (_I, [_P, _I]),
+    "maple_set_variant": (_I, [_P, _I, _I]),
     "maple_homogenize": (_I, [_P, _P, _I, _D, _I, _P, _P, _IP, _P]),
     "maple_sensitivity": (_I, [_P, _P, _P, _P, _P]),
     "maple_filter_box": (_I, [_P, _P, _P, _I, _P]),

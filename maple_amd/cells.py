@@ -117,6 +117,18 @@ class CellBatch:
         """Flexible (Polak-Ribiere) CG on/off; None = only in the mixed form."""
         L.check(self.lib.maple_set_cg(self._h, -1 if flexible is None else int(bool(flexible))))
 
+    def set_mixed_fallback(self, iters: int = 100):
+        """Mixed form: finish a solve on the FP64 V-cycle once it has run `iters` CG
+        iterations (0 = never)."""
+        L.check(self.lib.maple_set_mixed_fallback(self._h, int(iters)))
+
+    def set_variant(self, kernel: int, variant: int):
+        L.check(self.lib.maple_set_variant(self._h, int(kernel), int(variant)))
+
+    def set_coarse_refresh(self, every: int = 1):
+        """Rebuild the coarsest dense inverse every `every` homogenizations (1 = always)."""
+        L.check(self.lib.maple_set_coarse_refresh(self._h, int(every)))
+
     def set_element_matrices(self, cell: int, Ke24=None, Fe24=None, X0_24=None, Ke8=None, Fe8=None, X0_8=None):
         def p(a):
             if a is None:

@@ -81,6 +81,7 @@ struct Phys {
   Launch lf;               // level-0 operator launch, same
   double* atol2 = nullptr;
   bool have_fields = false, have_energy = false;
+  long long setups = 0;    // hierarchy builds so far (drives the coarsest-inverse refresh)
 };
 
 struct Timer {
@@ -101,6 +102,8 @@ struct maple_ctx {
   int warm = 0;
   int smoother = MAPLE_SMOOTH_L1_JACOBI;
   int flexible = -1;   // -1: flexible CG in the mixed form only
+  int coarse_refresh = 1;   // rebuild the coarsest inverse every this many homogenizations
+  int fallback = 100;       // mixed form: FP64 V-cycle after this many CG iterations (0: never)
   OCState* ocst = nullptr;
   OCState* ocst_h = nullptr;
   double* oc_a = nullptr;
@@ -203,9 +206,10 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
     }
     return 0;
   };
-  if (c->prec == MAPLE_PREC_FP64) {
-    TRY(alloc_vc(P.v64));
-  } else {
+  // the mixed form keeps an FP64 hierarchy too: it takes over a solve whose FP32
+  // preconditioner stagnates
+  TRY(alloc_vc(P.v64));
+  if (c->prec != MAPLE_PREC_FP64) {
     TRY(alloc_vc(P.v32));
     TRY(dalloc(c, &P.v32.Ke, (size_t)c->ncells * ne * ne));
     TRY(dalloc(c, &P.v32.rho, (size_t)c->lv[0].Ntot));
@@ -318,23 +322,33 @@ void apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const T* u, const
 }
 
 // Hierarchy of one density field: Jacobi diagonals, Galerkin stencils, dense coarsest inverse.
+// force_dense: rebuild the coarsest inverse now, outside the refresh cadence.
 template <typename T>
-void setup_t(maple_ctx* c, Phys& P, VC<T>& V, const double* rho, hipStream_t s) {
+void setup_t(maple_ctx* c, Phys& P, VC<T>& V, const double* rho, hipStream_t s, bool force_dense = false) {
   const int L = c->nlevels;
   const bool l1 = c->smoother == MAPLE_SMOOTH_L1_JACOBI;
   fine_dinv<T>(c->lv[0].node, P.nd, l1, P.Ke, rho, V.dinv[0], s);
   stencil_level1<T>(c->lv[1].node, c->lv[0].node, P.nd, P.G, rho, V.S[1], s);
   for (int l = 2; l < L; ++l) stencil_galerkin<T>(c->lv[l].node, c->lv[l - 1].node, P.nd, V.S[l - 1], V.S[l], s);
   for (int l = 1; l < L - 1; ++l) stencil_dinv<T>(c->lv[l].node, P.nd, l1, V.S[l], V.dinv[l], s);
-  dense_assemble_g<T>(c->lv[L - 1].node, P.ddc, P.maxn, P.nd, V.S[L - 1], P.A, P.gamma, s);
-  dense_inverse(P.ddc, c->ncells, P.max_nt, P.A, P.work, s);
+  // A lagged coarsest inverse (an earlier density's SPD inverse) keeps the V-cycle SPD;
+  // CG still stops at the same residual tolerance.
+  if (force_dense || P.setups % c->coarse_refresh == 0) {
+    dense_assemble_g<T>(c->lv[L - 1].node, P.ddc, P.maxn, P.nd, V.S[L - 1], P.A, P.gamma, s);
+    dense_inverse(P.ddc, c->ncells, P.max_nt, P.A, P.work, s);
+  }
+  if (!force_dense) P.setups += 1;
+}
+
+void setup_fp64(maple_ctx* c, Phys& P, const double* rho, hipStream_t s, bool force_dense) {
+  P.v64.Ke = P.Ke;
+  P.v64.rho = const_cast<double*>(rho);
+  setup_t<double>(c, P, P.v64, rho, s, force_dense);
 }
 
 void setup_hierarchy(maple_ctx* c, Phys& P, const double* rho, hipStream_t s) {
   if (c->prec == MAPLE_PREC_FP64) {
-    P.v64.Ke = P.Ke;
-    P.v64.rho = const_cast<double*>(rho);
-    setup_t<double>(c, P, P.v64, rho, s);
+    setup_fp64(c, P, rho, s, false);
   } else {
     vec_copy<float, double>(c->lv[0].Ntot, rho, P.v32.rho, s);
     setup_t<float>(c, P, P.v32, rho, s);
@@ -367,9 +381,10 @@ T* vcycle_t(maple_ctx* c, Phys& P, VC<T>& V, int l, const T* b, hipStream_t s) {
   return x;
 }
 
-// Preconditioner on the CG residual r (FP64 in, FP64 out).
-double* precondition(maple_ctx* c, Phys& P, const double* r, hipStream_t s) {
-  if (c->prec == MAPLE_PREC_FP64) return vcycle_t<double>(c, P, P.v64, 0, r, s);
+// Preconditioner on the CG residual r (FP64 in, FP64 out): the FP64 V-cycle, or the
+// FP32 one between conversions.
+double* precondition(maple_ctx* c, Phys& P, const double* r, bool use64, hipStream_t s) {
+  if (use64) return vcycle_t<double>(c, P, P.v64, 0, r, s);
   const long long n0 = (long long)P.nc * P.nd * c->lv[0].Ntot;
   vec_copy<float, double>(n0, r, P.v32.B[0], s);
   float* z = vcycle_t<float>(c, P, P.v32, 0, P.v32.B[0], s);
@@ -396,6 +411,7 @@ int enqueue_poll(maple_ctx* c, Phys& P, int slot, hipStream_t s) {
 int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStream_t s) {
   Level& L0 = c->lv[0];
   const bool flex = c->flexible < 0 ? c->prec != MAPLE_PREC_FP64 : c->flexible != 0;
+  bool use64 = c->prec == MAPLE_PREC_FP64;
   const long long n0 = (long long)P.nc * P.nd * L0.Ntot;
   vec_zero<int>((long long)c->ncells * P.nc, P.done_d, s);   // no case is converged before the start
   fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
@@ -409,7 +425,7 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   }
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, c->partial, s);
   reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
-  double* z = precondition(c, P, P.r, s);
+  double* z = precondition(c, P, P.r, use64, s);
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, c->partial, s);
   reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RZ_NEW, s);
   pcg_init_scalars(c->ncells, P.nc, rtol, P.atol2, P.sc, P.done_d, s);
@@ -431,7 +447,14 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
         vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, c->partial, s);
         reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RZO, s);
       }
-      z = precondition(c, P, P.r, s);
+      // An FP32 V-cycle cannot resolve the soft modes of solid islands held by 1e-9
+      // voids: a mixed solve still running after `fallback` iterations finishes on
+      // the FP64 hierarchy of the same density (flexible CG absorbs the switch).
+      if (!use64 && c->fallback > 0 && it + 1 == c->fallback) {
+        setup_fp64(c, P, rho, s, true);
+        use64 = true;
+      }
+      z = precondition(c, P, P.r, use64, s);
       vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, c->partial, s);
       reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RZ_NEW, s);
       pcg_beta(c->ncells, P.nc, flex, P.sc, s);
@@ -610,6 +633,30 @@ int maple_set_solver(maple_ctx* c, double omega, int nsmooth, int warm_start) {
   c->omega = omega;
   c->nsmooth = nsmooth;
   c->warm = warm_start ? 1 : 0;
+  return 0;
+}
+
+int maple_set_variant(maple_ctx* c, int kernel, int variant) {
+  if (!c) return fail(MAPLE_EINVAL, "null context");
+  if (kernel != MAPLE_KERNEL_FINE_OPERATOR) return fail(MAPLE_EINVAL, "unknown kernel id");
+  if (variant < 0 || variant > 1) return fail(MAPLE_EINVAL, "unknown variant");
+  c->lv[0].fine.var = variant;
+  for (Phys& P : c->ph) P.lf.var = variant;
+  return 0;
+}
+
+int maple_set_coarse_refresh(maple_ctx* c, int every) {
+  if (!c) return fail(MAPLE_EINVAL, "null context");
+  if (every < 1) return fail(MAPLE_EINVAL, "refresh interval must be >= 1");
+  c->coarse_refresh = every;
+  for (Phys& P : c->ph) P.setups = 0;
+  return 0;
+}
+
+int maple_set_mixed_fallback(maple_ctx* c, int iters) {
+  if (!c) return fail(MAPLE_EINVAL, "null context");
+  if (iters < 0) return fail(MAPLE_EINVAL, "fallback iteration count must be >= 0");
+  c->fallback = iters;
   return 0;
 }
 

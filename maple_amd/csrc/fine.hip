@@ -68,7 +68,11 @@ int fine_waves(const LevelCell& c) {
 
 // MODE: 0 = v = K u, 1 = v = K u and partial (u, K u), 2 = v = f - K u,
 //       3 = v = u + omega Dinv (f - K u)  (weighted Jacobi sweep)
-template <int ND, typename T, int MODE>
+// VAR 0: the compiler may keep all 18*ND neighbour values live at once;
+// VAR 1: a compiler barrier after each element stages the loads element by element
+//        (4 x 8*ND loads from L1 instead of 18*ND), trading re-loads for fewer live
+//        registers and more waves per SIMD.
+template <int ND, typename T, int MODE, int VAR>
 __global__ __launch_bounds__(256) void k_fine_apply(
     const LevelCell* __restrict__ cells, const int* __restrict__ blk_start, int ncells,
     long long Ntot, int ncase, const T* __restrict__ Ke, const T* __restrict__ rho,
@@ -158,6 +162,7 @@ __global__ __launch_bounds__(256) void k_fine_apply(
           if (rr == 0) lo[0] += t; else up[0] += t;
         }
       }
+      if constexpr (VAR == 1) asm volatile("" ::: "memory");
     }
   }
   double dj = 0.0;
@@ -182,16 +187,23 @@ __global__ __launch_bounds__(256) void k_fine_apply(
   }
 }
 
+template <int ND, typename T, int VAR>
+void fine_apply_v(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u,
+                  const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s) {
+  dim3 grid(L.nblocks, ncase), block(256);
+  switch (mode) {
+    case 0: k_fine_apply<ND, T, 0, VAR><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
+    case 1: k_fine_apply<ND, T, 1, VAR><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
+    case 2: k_fine_apply<ND, T, 2, VAR><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
+    default: k_fine_apply<ND, T, 3, VAR><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
+  }
+}
+
 template <int ND, typename T>
 void fine_apply(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u,
                 const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s) {
-  dim3 grid(L.nblocks, ncase), block(256);
-  switch (mode) {
-    case 0: k_fine_apply<ND, T, 0><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
-    case 1: k_fine_apply<ND, T, 1><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
-    case 2: k_fine_apply<ND, T, 2><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
-    default: k_fine_apply<ND, T, 3><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
-  }
+  if (L.var == 1) fine_apply_v<ND, T, 1>(L, mode, ncase, Ke, rho, u, f, dinv, omega, out, partial, s);
+  else fine_apply_v<ND, T, 0>(L, mode, ncase, Ke, rho, u, f, dinv, omega, out, partial, s);
 }
 
 template void fine_apply<3, double>(const Launch&, int, int, const double*, const double*, const double*, const double*, const double*, double, double*, double*, hipStream_t);

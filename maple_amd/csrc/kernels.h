@@ -16,6 +16,7 @@ struct Launch {
   // still iterating
   const int* done = nullptr;
   int ndc = 0;
+  int var = 0;   // kernel variant (level-0 operator: register staging), picked by the tuner
 };
 
 __device__ __forceinline__ bool case_done(const int* done, int ndc, int ci, int j) {

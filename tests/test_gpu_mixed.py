@@ -70,3 +70,31 @@ def test_mixed_vs_fp64_along_oc_trajectory():
             worst = max(worst, e, ek)
             print(f"OC it {it}: C_H rel err {e:.2e}  kappa rel err {ek:.2e}", flush=True)
     assert worst <= 1e-5
+
+
+@pytest.mark.parametrize("fallback", [10, 100])
+def test_mixed_islands_fall_back_to_fp64(fallback):
+    """Solid islands floating in 1e-9 void have soft modes that an FP32 V-cycle cannot
+    resolve; the mixed form finishes such solves on the FP64 hierarchy and still
+    matches the FP64 path (itself pinned to the oracle) solved tightly."""
+    import maple_amd
+    r = 16
+    print(f"[mixed islands] r={r} fallback={fallback}", flush=True)
+    x = np.full((r, r, r), 1e-3)
+    x[0, :, :] = 1.0          # a periodic wall keeps the solid connected
+    x[:, 0, :] = 1.0
+    rng = np.random.default_rng(11)
+    for _ in range(12):       # isolated voxels and small blobs away from the walls
+        i, j, k = rng.integers(3, r - 3, size=3)
+        x[i:i + rng.integers(1, 3), j:j + rng.integers(1, 3), k] = 1.0
+    rho = 1e-9 + (1 - 1e-9) * x.ravel() ** 3
+    cm = maple_amd.CellBatch([r], nlevels=3, device=DEV, precision="mixed")
+    cm.set_mixed_fallback(fallback)
+    cf = maple_amd.CellBatch([r], nlevels=3, device=DEV)
+    CH, kap, it = cm.homogenize(_t(rho), rtol=1e-6, maxit=3000)
+    CHf, kf, itf = cf.homogenize(_t(rho), rtol=1e-10, maxit=6000)
+    print(f"mixed iters {it.tolist()}  fp64 iters {itf.tolist()}", flush=True)
+    assert np.linalg.norm(CH[0].cpu().numpy() - CHf[0].cpu().numpy()) <= 1e-5 * np.linalg.norm(CHf[0].cpu().numpy())
+    assert np.linalg.norm(kap[0].cpu().numpy() - kf[0].cpu().numpy()) <= 1e-5 * np.linalg.norm(kf[0].cpu().numpy())
+    with pytest.raises(ValueError):
+        cm.set_mixed_fallback(-1)
