@@ -1,19 +1,28 @@
-"""Benchmark: OC iterations per second of the periodic inverse-homogenization loop.
+"""Benchmark: OC iterations per second of the periodic inverse-homogenization loop at 64^3.
 
-Workload (plain FP64 on one GPU): one 64^3 periodic cell, E0=1, nu=0.3,
-Emin=kmin=1e-9, SIMP p=3, box filter k=3 on dJ/dx, 6 elastic unit-strain +
-3 thermal unit-gradient cell problems, 4-level multigrid-preconditioned CG to
-rtol 1e-6, J = 0.5 stiffness + 0.5 kappa, volume fraction 0.25 on mean(x),
-move 0.2, x0 = 0.25 + U(-0.05, 0.05) (numpy default_rng(0)).  One step = one
-OC iteration: both homogenizations, sensitivities, filter and OC update.  The
-timed region is the whole 100-iteration trajectory from x0 (after a separate
-warm-up run), so it includes the hard middle of the trajectory where many
-small disconnected islands slow the cell solves down.
+Workload: 64^3 periodic cells, E0=1, nu=0.3, Emin=kmin=1e-9, SIMP p=3, box
+filter k=3 on dJ/dx, 6 elastic unit-strain + 3 thermal unit-gradient cell
+problems, 4-level multigrid-preconditioned CG to rtol 1e-6, J = 0.5 stiffness +
+0.5 kappa, volume fraction 0.25 on mean(x), move 0.2, x0 = 0.25 + U(-0.05, 0.05)
+(numpy default_rng(seed), cell k has seed k).  One OC iteration of a cell = both
+homogenizations, sensitivities, filter and OC update.  Every timed run is a
+whole 100-iteration trajectory from x0, including the hard middle where small
+disconnected islands slow the cell solves down.
 
-Prints one JSON line.  ``roofline`` is measured live for the dominant kernel
-(the level-0 elastic operator) with HIP events on the stream it runs on.
-``cpu_baseline`` times the FP64 CPU oracle (a restatement of the reference's
-algorithm) on the same 64^3 workload for a bounded sample of OC iterations.
+Forms (one JSON line; ``value`` is the headline form):
+* ``plain``    one cell, FP64, zero start every solve: the reference's method on the GPU.
+* ``headline`` 16 cells advanced together (one set of launches per OC step),
+               FP64 CG around an FP32 multigrid preconditioner, each solve
+               starting from the best of zero, the previous iterate's fields and
+               the learned network's prediction.  value = cells x iterations / s.
+* optional (--forms): ``fp64-warm``, ``mixed-warm`` (one cell), ``config4`` (the
+  64-cell mixed-resolution batch, 64^3-equivalent iterations / s).
+
+Accuracy: the last iterate's C_H and kappa of the headline form (every cell)
+against a tight (rtol 1e-11) FP64 solve of the same densities.  ``roofline`` is
+measured live for the dominant kernel (the level-0 elastic operator of the
+headline form's preconditioner) with HIP events on the stream it runs on.
+``cpu_baseline`` times the FP64 CPU oracle on one OC iteration of the same cell.
 """
 from __future__ import annotations
 
@@ -31,69 +40,93 @@ METRIC = "OC iters/sec at 64^3 (mixed+batched+learned vs plain FP64 GPU) at C_H 
 R = 64
 NLEVELS = 4
 OBJ = "stiffness:0.5,thermal:0.5"
-PEAK_FP64_TFLOPS = 78.6
-PEAK_FP32_TFLOPS = 157.3
+PEAK = {"fp64": 78.6, "mixed": 157.3}
 FLOP_PER_ELEM_ELASTIC = 1152      # 24 rows x 24 cols x 2 per element and load case
 
 
-def run_gpu(steps: int, warmup: int, profile_kernel: bool = True, warm_start: bool = False, maxit: int = 5000,
-            precision: str = "fp64", objective: str = OBJ):
-    """Warm up on `warmup` OC iterations from x0, then restart from x0 and time `steps` iterations."""
+def _x0(ncells):
+    import numpy as np
+
+    import maple_amd
+    return np.concatenate([maple_amd.x0_uniform_noise(R, seed=k) for k in range(ncells)])
+
+
+def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", warm: bool = False,
+             learned: bool = False, maxit: int = 5000, profile_kernel: bool = True):
     import numpy as np
     import torch
 
     import maple_amd
 
     dev = torch.device("cuda", 0)
-    cells = maple_amd.CellBatch([R], nlevels=NLEVELS, device=dev, warm_start=warm_start, precision=precision)
-    drv = maple_amd.OCDriver(cells, objective, 0.25, move=0.2, xmin=1e-3, filter_k=3, rtol=1e-6, maxit=maxit)
-    x0 = torch.from_numpy(maple_amd.x0_uniform_noise(R, seed=0)).to(dev)
-    x = x0.clone()
-    for _ in range(warmup):
-        x, info = drv.step(x)
-    torch.cuda.synchronize()
+
+    def make():
+        cells = maple_amd.CellBatch([R] * ncells, nlevels=NLEVELS, device=dev, warm_start=warm, precision=precision)
+        start = None
+        if learned:
+            from maple_amd import learned as ml
+            start = ml.LearnedStart(cells, ml.UNet.load(max_batch=min(16, ncells), device=dev))
+        drv = maple_amd.OCDriver(cells, OBJ, 0.25, move=0.2, xmin=1e-3, filter_k=3, rtol=1e-6, maxit=maxit,
+                                 start=start)
+        return cells, drv
+
+    x0 = torch.from_numpy(_x0(ncells)).to(dev)
+    if warmup:
+        cells, drv = make()
+        x = x0.clone()
+        for _ in range(warmup):
+            x, _ = drv.step(x)
+        torch.cuda.synchronize()
+        cells.close()
+    cells, drv = make()
     kind = 0 if precision == "fp64" else 2
     if profile_kernel:
         cells.timing(True)
         cells.timing_read(kind)
     iters = []
     x = x0.clone()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         x, info = drv.step(x)
-        iters.append(info["iters"][0].tolist())
+        iters.append(info["iters"].max(axis=0).tolist())
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     count, kms = cells.timing_read(kind) if profile_kernel else (0, 0.0)
     cells.timing(False)
-    # accuracy of the last iterate's C_H and kappa against a tight FP64 solve of the same rho
-    rho = drv.design_to_rho(x)
-    cells.set_solver(warm_start=False)
-    CH6, k6, _ = cells.homogenize(rho, ("elastic", "thermal"), rtol=1e-6, maxit=maxit)
-    ref = maple_amd.CellBatch([R], nlevels=NLEVELS, device=dev)
+    # accuracy: the last step's C_H and kappa (solved at info["rho"]) against a tight FP64 solve
+    rho = info["rho"]
+    ref = maple_amd.CellBatch([R] * ncells, nlevels=NLEVELS, device=dev)
     CHt, kt, _ = ref.homogenize(rho, ("elastic", "thermal"), rtol=1e-11, maxit=4 * maxit, strict=False)
-    err = max(float(torch.linalg.norm(CH6 - CHt) / torch.linalg.norm(CHt)),
-              float(torch.linalg.norm(k6 - kt) / torch.linalg.norm(kt)))
+    CH6 = torch.from_numpy(info["CH"]).to(dev)
+    k6 = torch.from_numpy(info["kappa"]).to(dev)
+    err = 0.0
+    for c in range(ncells):
+        err = max(err, float(torch.linalg.norm(CH6[c] - CHt[c]) / torch.linalg.norm(CHt[c])),
+                  float(torch.linalg.norm(k6[c] - kt[c]) / torch.linalg.norm(kt[c])))
     ref.close()
-    ms = (t1 - t0) * 1e3 / steps
+    src = None
+    if learned or warm:
+        src = {p: np.bincount(cells.start_sources(p).ravel(), minlength=3).tolist() for p in ("elastic", "thermal")}
+    cells.close()
+    dt = t1 - t0
     it = np.array(iters)
-    out = {"ms_per_step": ms, "value": 1e3 / ms, "iters_el_max": int(it[:, :6].max()), "iters_th_max": int(it[:, 6:].max()),
-           "iters_el_mean": float(it[:, :6].max(axis=1).mean()), "iters_th_mean": float(it[:, 6:].max(axis=1).mean()),
-           "iters_per_step": it[:, [0, 6]].tolist(), "J_last": float(info["J"][0]), "ch_rel_err": err,
-           "mean_x": float(x.mean())}
+    out = {"value": ncells * steps / dt, "ms_per_step": dt * 1e3 / steps, "cells": ncells,
+           "iters_el_max": int(it[:, :6].max()), "iters_el_mean": float(it[:, :6].max(axis=1).mean()),
+           "iters_th_mean": float(it[:, 6:].max(axis=1).mean()), "ch_rel_err": err, "J_last": float(info["J"][0]),
+           "last_starts": src}
     if count:
         avg_s = kms / count * 1e-3
-        flops = 6 * R ** 3 * FLOP_PER_ELEM_ELASTIC
+        flops = ncells * 6 * R ** 3 * FLOP_PER_ELEM_ELASTIC
         tname = "double" if precision == "fp64" else "float"
-        out["kernel"] = {"name": f"k_fine_apply<3,{tname},*> (level-0 elastic operator, 6 load cases)",
+        out["kernel"] = {"name": f"k_fine_apply<3,{tname},*> (level-0 elastic operator, 6 load cases x {ncells} cells)",
                          "launches": count, "avg_us": avg_s * 1e6, "flop_per_launch": flops,
-                         "achieved_tflops": flops / avg_s / 1e12,
-                         "peak": PEAK_FP64_TFLOPS if precision == "fp64" else PEAK_FP32_TFLOPS}
+                         "achieved_tflops": flops / avg_s / 1e12, "peak": PEAK[precision]}
     return out
 
 
-def run_batched(steps: int, warmup: int, precision: str = "fp64", maxit: int = 5000):
-    """The 64-cell mixed-resolution batch: every OC step advances all cells with one set of launches."""
+def run_config4(steps: int, precision: str = "mixed", maxit: int = 5000):
+    """The 64-cell mixed-resolution batch (16 x 32^3, 32 x 48^3, 16 x 64^3)."""
     import numpy as np
     import torch
 
@@ -101,34 +134,24 @@ def run_batched(steps: int, warmup: int, precision: str = "fp64", maxit: int = 5
 
     dev = torch.device("cuda", 0)
     spec = maple_amd.batched_workload()
-    cells = maple_amd.CellBatch([c["r"] for c in spec], nlevels=NLEVELS, device=dev, precision=precision)
+    cells = maple_amd.CellBatch([c["r"] for c in spec], nlevels=NLEVELS, device=dev, precision=precision,
+                                warm_start=True)
     drv = maple_amd.OCDriver(cells, [c["weights"] for c in spec], [c["volfrac"] for c in spec], move=0.2, xmin=1e-3,
                              filter_k=3, rtol=1e-6, maxit=maxit)
-    x0 = torch.from_numpy(np.concatenate([c["x0"] for c in spec])).to(dev)
-    x = x0.clone()
-    for _ in range(warmup):
-        x, info = drv.step(x)
+    x = torch.from_numpy(np.concatenate([c["x0"] for c in spec])).to(dev)
     torch.cuda.synchronize()
-    x = x0.clone()
-    iters = []
     t0 = time.perf_counter()
     for _ in range(steps):
         x, info = drv.step(x)
-        iters.append(info["iters"].max(axis=0).tolist())
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     elems = sum(c["r"] ** 3 for c in spec)
-    it = np.array(iters)
-    return {"steps_per_s": steps / dt, "ms_per_step": dt * 1e3 / steps, "cells": len(spec), "elements": elems,
-            "cell_oc_iters_per_s": len(spec) * steps / dt,
-            "equiv_64cube_oc_iters_per_s": elems / R ** 3 * steps / dt,
-            "elastic_cg_mean_of_step_max": float(it[:, :6].max(axis=1).mean()),
-            "thermal_cg_mean_of_step_max": float(it[:, 6:].max(axis=1).mean())}
+    cells.close()
+    return {"steps": steps, "ms_per_step": dt * 1e3 / steps, "cells": len(spec), "elements": elems,
+            "cell_oc_iters_per_s": len(spec) * steps / dt, "equiv_64cube_oc_iters_per_s": elems / R ** 3 * steps / dt}
 
 
 def cpu_baseline(n_iter: int):
-    import numpy as np
-
     from oracle import driver
 
     cores = len(os.sched_getaffinity(0))
@@ -140,8 +163,18 @@ def cpu_baseline(n_iter: int):
     total = time.perf_counter() - t0
     per = sum(times) / len(times)
     return {"value": 1.0 / per, "unit": "OC iters/sec", "cores": cores, "kind": "port",
-            "sample": f"{n_iter} OC iteration(s) of the same 64^3 workload from x0, FP64 numpy/scipy oracle, "
-                      f"{total:.1f} s wall"}
+            "sample": f"{n_iter} OC iteration(s) of the same 64^3 workload from x0, FP64 numpy/scipy oracle "
+                      f"(multithreaded BLAS/sparse kernels), {total:.1f} s wall"}
+
+
+FORMS = {
+    "plain": dict(ncells=1, precision="fp64", warm=False, learned=False),
+    "headline": dict(ncells=16, precision="mixed", warm=True, learned=True),
+    "fp64-warm": dict(ncells=1, precision="fp64", warm=True, learned=False),
+    "mixed-warm": dict(ncells=1, precision="mixed", warm=True, learned=False),
+    "mixed-warm-learned": dict(ncells=1, precision="mixed", warm=True, learned=True),
+    "batched-mixed-warm": dict(ncells=16, precision="mixed", warm=True, learned=False),
+}
 
 
 def main():
@@ -149,26 +182,28 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--forms", default="fp64,mixed", help="comma list of fp64, mixed")
-    ap.add_argument("--batched", default="", help="also run the 64-cell batch in these precisions (fp64,mixed)")
-    ap.add_argument("--batched-steps", type=int, default=20)
-    ap.add_argument("--warm-start", action="store_true", help="start each CG from the previous OC iterate's fields")
+    ap.add_argument("--forms", default="plain,headline", help="comma list of " + ", ".join(FORMS))
+    ap.add_argument("--config4", action="store_true", help="also run the 64-cell mixed-resolution batch")
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (cells shard by process "
-              "per GPU with no collectives, see DESIGN.md); --gpus above 1 is not supported.", file=sys.stderr)
+              "per GPU with no collectives); --gpus above 1 is not supported.", file=sys.stderr)
         sys.exit(2)
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    for f in forms:
+        if f not in FORMS:
+            raise SystemExit(f"unknown form {f}")
     res = {}
     for f in forms:
-        res[f] = run_gpu(args.steps, args.warmup, warm_start=args.warm_start, precision=f)
-        print(f"# {f}: {res[f]['value']:.3f} OC it/s, C_H/kappa rel err {res[f]['ch_rel_err']:.2e}", file=sys.stderr,
-              flush=True)
+        res[f] = run_form(args.steps, args.warmup, **FORMS[f])
+        print(f"# {f}: {res[f]['value']:.3f} 64^3 OC it/s, C_H/kappa rel err {res[f]['ch_rel_err']:.2e}, "
+              f"elastic CG mean {res[f]['iters_el_mean']:.1f}", file=sys.stderr, flush=True)
     ok = [f for f in forms if res[f]["ch_rel_err"] <= 1e-5]
-    head = max(ok, key=lambda f: res[f]["value"]) if ok else forms[0]
+    head = "headline" if "headline" in ok else (max(ok, key=lambda f: res[f]["value"]) if ok else forms[0])
     g = res[head]
+    spec = FORMS[head]
     line = {
         "metric": METRIC,
         "value": g["value"],
@@ -180,23 +215,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == "fp64" else "f64+f32",
+        "dtype": "f64" if spec["precision"] == "fp64" else "f64+f32 (+bf16 network)" if spec["learned"] else "f64+f32",
         "data": "synthetic",
-        "config": {"workload": "one 64^3 periodic cell, 100 OC iterations from x0=0.25+U(-0.05,0.05) seed 0, "
-                               "J=0.5E+0.5kappa, vf 0.25, move 0.2, filter k=3, 4-level MG-PCG rtol 1e-6",
-                   "form": {"fp64": "plain FP64", "mixed": "FP64 CG + FP32 multigrid preconditioner"}[head],
-                   "cells": 1, "resolution": R, "levels": NLEVELS, "rtol": 1e-6,
-                   "warm_start": bool(args.warm_start)},
-        "cg": {"elastic_max": g["iters_el_max"], "thermal_max": g["iters_th_max"],
-               "elastic_mean_of_step_max": g["iters_el_mean"], "thermal_mean_of_step_max": g["iters_th_mean"]},
-        "J_last": g["J_last"],
+        "config": {"workload": f"{spec['ncells']} x 64^3 periodic cell(s), 100 OC iterations each from "
+                               "x0=0.25+U(-0.05,0.05) (seed = cell index), J=0.5E+0.5kappa, vf 0.25, move 0.2, "
+                               "filter k=3, 4-level MG-PCG rtol 1e-6",
+                   "form": head, "cells": spec["ncells"], "precision": spec["precision"],
+                   "warm_start": spec["warm"], "learned_start": spec["learned"], "resolution": R,
+                   "levels": NLEVELS, "rtol": 1e-6},
         "ch_rel_err_vs_fp64_tight_solve": g["ch_rel_err"],
-        "forms": {f: {"value": res[f]["value"], "ms_per_step": res[f]["ms_per_step"],
-                      "ch_rel_err": res[f]["ch_rel_err"], "elastic_cg_mean": res[f]["iters_el_mean"],
-                      "kernel_avg_us": res[f].get("kernel", {}).get("avg_us")} for f in forms},
+        "forms": {f: {k: res[f][k] for k in ("value", "ms_per_step", "cells", "ch_rel_err", "iters_el_mean",
+                                             "iters_th_mean", "iters_el_max", "last_starts")}
+                  | {"kernel_avg_us": res[f].get("kernel", {}).get("avg_us")} for f in forms},
     }
-    if "fp64" in res and head != "fp64":
-        line["speedup_vs_plain_fp64"] = g["value"] / res["fp64"]["value"]
+    if "plain" in res and head != "plain":
+        line["speedup_vs_plain_fp64"] = g["value"] / res["plain"]["value"]
     k = g.get("kernel")
     if k:
         line["roofline"] = {"bound": "mfma", "achieved": k["achieved_tflops"], "peak": k["peak"],
@@ -205,8 +238,8 @@ def main():
                             "algorithmic_flop_per_launch": k["flop_per_launch"],
                             "roof": "vector FMA and MFMA share the MI355X peak for this precision "
                                     "(FP64 78.6, FP32 157.3 TFLOP/s)"}
-    for p in [q.strip() for q in args.batched.split(",") if q.strip()]:
-        line.setdefault("batched", {})[p] = run_batched(args.batched_steps, 1, precision=p)
+    if args.config4:
+        line["config4"] = run_config4(args.steps)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_iters)
     print(json.dumps(line), flush=True)

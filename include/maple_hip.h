@@ -20,6 +20,8 @@
 #ifndef MAPLE_HIP_H
 #define MAPLE_HIP_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -155,8 +157,26 @@ int maple_apply_operator(maple_ctx* ctx, int physics, const double* rho_dev, int
 int maple_get_fields(maple_ctx* ctx, int physics, double* u_dev, void* stream);
 int maple_set_fields(maple_ctx* ctx, int physics, const double* u_dev, void* stream);
 
-/* Sizes: total elements (= nodes) at level 0 over all cells. */
+/* Supply a start for the next solve of one physics (plane-major fields at level
+ * 0, as maple_get_fields).  Per (cell, case) that solve starts from whichever of
+ * zero, the previous fields (when warm start is on) and this start has the
+ * smallest residual ||f - K u||, so a poor start costs one operator application
+ * and nothing else (no counterpart in the reference: its warm-start hook
+ * init_list, EBE_GMG.py:648-757 and 852-858, is never fed, model.py:120). */
+int maple_set_start(maple_ctx* ctx, int physics, const double* u_dev, void* stream);
+
+/* Start chosen by the last solve of one physics, per (cell, case): 0 zero,
+ * 1 previous fields, 2 supplied start (ncells*6 or ncells*3 ints). */
+int maple_start_sources(maple_ctx* ctx, int physics, int* src_host);
+
+/* fine_dev = P coarse_dev: periodic trilinear prolongation of plane-major
+ * fields (all cases of one physics) from level `level` (>= 1) to level-1
+ * (ProlongStencil.prolong, EBE_GMG.py:38-58). */
+int maple_prolong(maple_ctx* ctx, int physics, int level, const double* coarse_dev, double* fine_dev, void* stream);
+
+/* Sizes: total elements (= nodes) at level 0 over all cells; nodes at a level. */
 long long maple_num_elements(const maple_ctx* ctx);
+long long maple_num_nodes(const maple_ctx* ctx, int level);
 
 /* Kernel timing with HIP events on the call's stream around every launch of
  * the level-0 operator kernels (enable, then read: count of launches and their
@@ -164,6 +184,46 @@ long long maple_num_elements(const maple_ctx* ctx);
  * thermal, 2: FP32 elastic (mixed form's preconditioner). */
 int maple_timing_enable(maple_ctx* ctx, int enable);
 int maple_timing_read(maple_ctx* ctx, int kind, long long* count, double* total_ms);
+
+/* ---- learned warm start ---------------------------------------------------
+ * A 3-level periodic 3-D U-Net (channels 16-32-64, 3x3x3 convolutions with
+ * circular padding, 2x2x2 mean pooling, nearest x2 upsampling, skip
+ * concatenation, 1x1x1 head) that maps a density feature on an n^3 grid to the
+ * 21 characteristic-field channels (elastic case j component c at 3*j + c,
+ * thermal case j at 18 + j), trained and evaluated by hand-written BF16 MFMA
+ * kernels with FP32 accumulation.  No counterpart in the reference: its
+ * warm-start hook `init_list` (EBE_GMG.py:648-757) is never fed (model.py:120,
+ * 149).  Tensors are channels-last: value (sample b, voxel (x,y,z), channel c)
+ * at ((b*n + x)*n + y)*n + z)*C + c.  Parameters are one FP32 vector: per layer
+ * in the order e1a e1b e2a e2b b1 b2 d2a d2b d1a d1b head, W[co][o][ci]
+ * (o = (dx+1)*9 + (dy+1)*3 + (dz+1)) then b[co]. */
+typedef struct maple_unet maple_unet;
+
+/* Parameter count of the network. */
+long long maple_unet_num_params(void);
+
+/* Network on `device` for grids of n^3 voxels (n a multiple of 4 in [16, 128])
+ * and batches of up to max_batch samples; every buffer is allocated here. */
+int maple_unet_create(int device, int n, int max_batch, maple_unet** out);
+int maple_unet_destroy(maple_unet* net);
+
+/* Copy the FP32 parameter vector in (resets the Adam state) or out. */
+int maple_unet_set_params(maple_unet* net, const float* theta_host);
+int maple_unet_get_params(maple_unet* net, float* theta_host);
+
+/* feat_dev: batch*n^3 FP32 -> out_dev: batch*n^3*21 FP32 (raw network output). */
+int maple_unet_forward(maple_unet* net, const float* feat_dev, int batch, float* out_dev, void* stream);
+
+/* Loss mean((y - t)^2) over batch*n^3*21 values against BF16 targets (raw bits,
+ * same layout as the output) and its gradient (FP32, parameter order) into
+ * grad_dev (may be NULL).  Synchronizes the stream to read the loss. */
+int maple_unet_loss_grad(maple_unet* net, const float* feat_dev, const uint16_t* target_dev, int batch,
+                         double* loss_host, float* grad_dev, void* stream);
+
+/* One Adam step (beta 0.9/0.999, eps 1e-8, learning rate lr) on the loss above;
+ * loss_host receives the loss before the step. */
+int maple_unet_train_step(maple_unet* net, const float* feat_dev, const uint16_t* target_dev, int batch, float lr,
+                          double* loss_host, void* stream);
 
 #ifdef __cplusplus
 }

@@ -55,9 +55,21 @@ SIGNATURES = {
     "maple_apply_operator": (_I, [_P, _I, _P, _I, _P, _P, _P]),
     "maple_get_fields": (_I, [_P, _I, _P, _P]),
     "maple_set_fields": (_I, [_P, _I, _P, _P]),
+    "maple_set_start": (_I, [_P, _I, _P, _P]),
+    "maple_start_sources": (_I, [_P, _I, _IP]),
+    "maple_prolong": (_I, [_P, _I, _I, _P, _P, _P]),
     "maple_num_elements": (C.c_longlong, [_P]),
+    "maple_num_nodes": (C.c_longlong, [_P, _I]),
     "maple_timing_enable": (_I, [_P, _I]),
     "maple_timing_read": (_I, [_P, _I, C.POINTER(C.c_longlong), _DP]),
+    "maple_unet_num_params": (C.c_longlong, []),
+    "maple_unet_create": (_I, [_I, _I, _I, C.POINTER(_P)]),
+    "maple_unet_destroy": (_I, [_P]),
+    "maple_unet_set_params": (_I, [_P, C.POINTER(C.c_float)]),
+    "maple_unet_get_params": (_I, [_P, C.POINTER(C.c_float)]),
+    "maple_unet_forward": (_I, [_P, _P, _I, _P, _P]),
+    "maple_unet_loss_grad": (_I, [_P, _P, _P, _I, _DP, _P, _P]),
+    "maple_unet_train_step": (_I, [_P, _P, _P, _I, C.c_float, _DP, _P]),
 }
 
 _lib = None

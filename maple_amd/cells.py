@@ -221,6 +221,30 @@ class CellBatch:
         u = self._vec(u, NCASE[physics] * NDOF[physics] * self.nelem, "u")
         L.check(self.lib.maple_set_fields(self._h, PHYS[physics], C.c_void_p(u.data_ptr()), self._stream()))
 
+    def set_start(self, physics: str, u: torch.Tensor):
+        """Candidate start of the next solve (ncase, ndof, Etot); per (cell, case) the solve
+        starts from the best of zero, the previous fields (warm start) and this."""
+        u = self._vec(u, NCASE[physics] * NDOF[physics] * self.nelem, "u")
+        L.check(self.lib.maple_set_start(self._h, PHYS[physics], C.c_void_p(u.data_ptr()), self._stream()))
+
+    def start_sources(self, physics: str) -> np.ndarray:
+        """Start chosen by the last solve, (ncells, ncase): 0 zero, 1 previous fields, 2 supplied."""
+        out = (C.c_int * (self.ncells * NCASE[physics]))()
+        L.check(self.lib.maple_start_sources(self._h, PHYS[physics], out))
+        return np.array(out, dtype=np.int32).reshape(self.ncells, NCASE[physics])
+
+    def num_nodes(self, level: int) -> int:
+        return int(self.lib.maple_num_nodes(self._h, int(level)))
+
+    def prolong(self, physics: str, level: int, coarse: torch.Tensor) -> torch.Tensor:
+        """Periodic trilinear prolongation of (ncase, ndof, N_level) fields to level - 1."""
+        nc, nd = NCASE[physics], NDOF[physics]
+        coarse = self._vec(coarse, nc * nd * self.num_nodes(level), "coarse")
+        fine = self._empty(nc, nd, self.num_nodes(level - 1))
+        L.check(self.lib.maple_prolong(self._h, PHYS[physics], int(level), C.c_void_p(coarse.data_ptr()),
+                                       C.c_void_p(fine.data_ptr()), self._stream()))
+        return fine
+
     # -- timing -------------------------------------------------------------
     def timing(self, enable: bool):
         L.check(self.lib.maple_timing_enable(self._h, int(bool(enable))))

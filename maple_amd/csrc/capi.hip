@@ -81,6 +81,8 @@ struct Phys {
   Launch lf;               // level-0 operator launch, same
   double* atol2 = nullptr;
   bool have_fields = false, have_energy = false;
+  double* ustart = nullptr;   // supplied start of the next solve
+  bool have_start = false;
   long long setups = 0;    // hierarchy builds so far (drives the coarsest-inverse refresh)
 };
 
@@ -221,6 +223,7 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
   TRY(dalloc(c, &P.p, n0));
   TRY(dalloc(c, &P.q, n0));
   TRY(dalloc(c, &P.f, n0));
+  TRY(dalloc(c, &P.ustart, n0));
   HIPCHK(hipMemset(P.u, 0, n0 * sizeof(double)));
   // dense coarsest level
   const Level& LC = c->lv[L - 1];
@@ -417,12 +420,21 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, c->partial, s);
   reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_FF, s);
-  if (c->warm && P.have_fields) {
+  // start: per (cell, case) the smallest residual among zero, the previous fields
+  // (warm start) and a supplied start (learned warm start)
+  const bool have_prev = c->warm && P.have_fields;
+  if (have_prev) {
     apply_fine64(c, P, 2, rho, P.u, P.f, P.r, nullptr, s);
-  } else {
-    vec_zero<double>(n0, P.u, s);
-    vec_copy<double, double>(n0, P.f, P.r, s);
+    vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, c->partial, s);
+    reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
   }
+  if (P.have_start) {
+    apply_fine64(c, P, 2, rho, P.ustart, P.f, P.q, nullptr, s);
+    vec_dot<double, double>(L0.node, P.nd, P.nc, P.q, P.q, c->partial, s);
+    reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RRS, s);
+  }
+  pcg_pick_start(L0.node, c->ncells, P.nd, P.nc, have_prev, P.have_start, P.sc, P.u, P.r, P.ustart, P.q, P.f, s);
+  P.have_start = false;
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, c->partial, s);
   reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
   double* z = precondition(c, P, P.r, use64, s);
@@ -477,6 +489,8 @@ int check_ctx(const maple_ctx* c) {
 }
 
 }  // namespace
+
+int maple_record_error(int code, const std::string& msg) { return fail(code, msg); }
 
 // ---------------------------------------------------------------------------
 
@@ -797,6 +811,43 @@ int maple_set_fields(maple_ctx* c, int physics, const double* u, void* stream) {
                         (hipStream_t)stream));
   P.have_fields = true;
   return 0;
+}
+
+int maple_set_start(maple_ctx* c, int physics, const double* u, void* stream) {
+  TRY(check_ctx(c));
+  if (physics != MAPLE_ELASTIC && physics != MAPLE_THERMAL) return fail(MAPLE_EINVAL, "physics must be one of ELASTIC, THERMAL");
+  Phys& P = c->ph[physics == MAPLE_ELASTIC ? 0 : 1];
+  if (!u) return fail(MAPLE_EINVAL, "null input");
+  HIPCHK(hipMemcpyAsync(P.ustart, u, sizeof(double) * P.nc * P.nd * c->lv[0].Ntot, hipMemcpyDeviceToDevice,
+                        (hipStream_t)stream));
+  P.have_start = true;
+  return 0;
+}
+
+int maple_start_sources(maple_ctx* c, int physics, int* src_host) {
+  if (!c || !src_host) return fail(MAPLE_EINVAL, "null argument");
+  if (physics != MAPLE_ELASTIC && physics != MAPLE_THERMAL) return fail(MAPLE_EINVAL, "physics must be one of ELASTIC, THERMAL");
+  const Phys& P = c->ph[physics == MAPLE_ELASTIC ? 0 : 1];
+  for (size_t i = 0; i < P.hsc.size(); ++i) src_host[i] = P.hsc[i].src;
+  return 0;
+}
+
+int maple_prolong(maple_ctx* c, int physics, int level, const double* coarse, double* fine, void* stream) {
+  TRY(check_ctx(c));
+  if (physics != MAPLE_ELASTIC && physics != MAPLE_THERMAL) return fail(MAPLE_EINVAL, "physics must be one of ELASTIC, THERMAL");
+  if (level < 1 || level >= c->nlevels) return fail(MAPLE_EINVAL, "level must be in [1, nlevels)");
+  if (!coarse || !fine) return fail(MAPLE_EINVAL, "null input");
+  const Phys& P = c->ph[physics == MAPLE_ELASTIC ? 0 : 1];
+  hipStream_t s = (hipStream_t)stream;
+  vec_zero<double>((long long)P.nc * P.nd * c->lv[level - 1].Ntot, fine, s);
+  prolong_add<double>(c->lv[level - 1].node, c->lv[level].node, P.nd, P.nc, coarse, fine, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+long long maple_num_nodes(const maple_ctx* c, int level) {
+  if (!c || level < 0 || level >= c->nlevels) return 0;
+  return c->lv[level].Ntot;
 }
 
 long long maple_num_elements(const maple_ctx* c) { return c ? c->lv[0].Ntot : 0; }

@@ -1,6 +1,8 @@
 // Kernel launchers of the cell-solver library (device code lives in *.hip).
 #pragma once
 
+#include <string>
+
 #include "common.h"
 
 // A launch over every cell of a batch: blk_start (device, ncells+1 entries)
@@ -28,6 +30,9 @@ __device__ __forceinline__ bool cell_done(const int* done, int ndc, int ci) {
     if (!done[ci * ndc + j]) return false;
   return true;
 }
+
+// Records the calling thread's last error (maple_last_error) and returns code.
+int maple_record_error(int code, const std::string& msg);
 
 // ---- level 0 (fine.hip) ---------------------------------------------------
 int fine_waves(const LevelCell& c);  // wavefronts per cell of k_fine_apply (4 per block)
@@ -85,14 +90,16 @@ struct CaseScalars {   // one per (cell, case)
   double tgt2;         // squared residual target
   double rz, rz_new, pq, rr;
   double rzo;          // (r_new, z_old): the flexible (Polak-Ribiere) correction of beta
+  double rrs;          // squared residual of a supplied start (learned warm start)
   double alpha, beta;
   int done, iters;
-  int pad_[2];
+  int src;             // start of the last solve: 0 zero, 1 previous fields, 2 supplied start
+  int pad_;
 };
 // sum partial[blk*6 + j] over each cell's blocks into dst (a member of CaseScalars)
 void reduce_partials(const int* blk_start, int ncells, int ncase, const double* partial,
                      CaseScalars* sc, int which, hipStream_t s);
-enum { SC_FF = 0, SC_RZ_NEW, SC_PQ, SC_RR, SC_RZO };
+enum { SC_FF = 0, SC_RZ_NEW, SC_PQ, SC_RR, SC_RZO, SC_RRS };
 // elementwise (cell-aligned 256-node blocks):
 template <typename T>
 void vec_scale_dinv(const Launch& L, int nd, int ncase, T omega, const T* dinv, const T* b, T* x, hipStream_t s);
@@ -102,6 +109,11 @@ void vec_dot(const Launch& L, int nd, int ncase, const Ta* a, const Tb* b, doubl
 // u += alpha p; r -= alpha q; partial (r, r)
 void pcg_update_ur(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* p, const double* q,
                    double* u, double* r, double* partial, hipStream_t s);
+// Start of a solve, per (cell, case): the candidate with the smallest residual
+// among zero (residual f), the previous fields u (residual r, if have_prev) and a
+// supplied start us (residual rs, if have_start); u and r receive the choice.
+void pcg_pick_start(const Launch& L, int ncells, int nd, int ncase, bool have_prev, bool have_start, CaseScalars* sc,
+                    double* u, double* r, const double* us, const double* rs, const double* f, hipStream_t s);
 // p = z + beta p
 void pcg_update_p(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* z, double* p, hipStream_t s);
 // per (cell, case) scalar steps

@@ -21,6 +21,7 @@ __global__ __launch_bounds__(64) void k_reduce_partials(const int* __restrict__ 
     else if (which == SC_RZ_NEW) s.rz_new = v;
     else if (which == SC_PQ) s.pq = v;
     else if (which == SC_RZO) s.rzo = v;
+    else if (which == SC_RRS) s.rrs = v;
     else s.rr = v;
   }
 }
@@ -130,6 +131,44 @@ __global__ __launch_bounds__(256) void k_update_p(Launch L, int nd, int ncase, c
 void pcg_update_p(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* z, double* p,
                   hipStream_t s) {
   k_update_p<<<L.nblocks, 256, 0, s>>>(L, nd, ncase, sc, z, p);
+}
+
+__global__ void k_pick_src(int n, int have_prev, int have_start, CaseScalars* __restrict__ sc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  CaseScalars& c = sc[t];
+  int src = 0;
+  double best = c.ff;
+  if (have_prev && c.rr < best) {
+    src = 1;
+    best = c.rr;
+  }
+  if (have_start && c.rrs < best) src = 2;
+  c.src = src;
+}
+
+__global__ __launch_bounds__(256) void k_apply_start(Launch L, int nd, int ncase, const CaseScalars* __restrict__ sc,
+                                                     double* __restrict__ u, double* __restrict__ r,
+                                                     const double* __restrict__ us, const double* __restrict__ rs,
+                                                     const double* __restrict__ f) {
+  int ci; long long gid;
+  if (!vnode(L, ci, gid)) return;
+  for (int j = 0; j < ncase; ++j) {
+    const int src = sc[ci * ncase + j].src;
+    if (src == 1) continue;
+    for (int i = 0; i < nd; ++i) {
+      const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
+      u[idx] = src == 2 ? us[idx] : 0.0;
+      r[idx] = src == 2 ? rs[idx] : f[idx];
+    }
+  }
+}
+
+void pcg_pick_start(const Launch& L, int ncells, int nd, int ncase, bool have_prev, bool have_start, CaseScalars* sc,
+                    double* u, double* r, const double* us, const double* rs, const double* f, hipStream_t s) {
+  const int n = ncells * ncase;
+  k_pick_src<<<(n + 63) / 64, 64, 0, s>>>(n, have_prev ? 1 : 0, have_start ? 1 : 0, sc);
+  k_apply_start<<<L.nblocks, 256, 0, s>>>(L, nd, ncase, sc, u, r, us, rs, f);
 }
 
 // ff and rr must already hold ||f||^2 and ||r0||^2.

@@ -171,7 +171,10 @@ class OCDriver:
 
     def __init__(self, cells: CellBatch, objectives: Sequence, volfrac: Sequence[float], penal: float = 3.0,
                  rho_min: float = 1e-9, move: float = 0.2, xmin: float = 1e-3, filter_k: int = 3,
-                 rtol: float = 1e-6, maxit: int = 1000):
+                 rtol: float = 1e-6, maxit: int = 1000, physics: Optional[Sequence[str]] = None, start=None):
+        """physics: solve these physics whatever the objectives need (None: what they need).
+        start: callable(rho, physics) that supplies start candidates before each
+        homogenization (e.g. maple_amd.learned.LearnedStart)."""
         self.cells = cells
         n = cells.ncells
         objs = list(objectives) if not isinstance(objectives, (str, dict)) else [objectives] * n
@@ -184,6 +187,11 @@ class OCDriver:
         self.need_el = any(t in ("stiffness", "shear", "bulk") for w in self.weights for t in w)
         self.need_th = any("thermal" in w for w in self.weights)
         self.physics = tuple(p for p, need in (("elastic", self.need_el), ("thermal", self.need_th)) if need)
+        if physics is not None:
+            self.physics = tuple(physics)
+            if (self.need_el and "elastic" not in self.physics) or (self.need_th and "thermal" not in self.physics):
+                raise ValueError("the objectives need a physics that is not solved")
+        self.start = start
 
     def design_to_rho(self, x: torch.Tensor) -> torch.Tensor:
         xu = x.clamp(self.xmin, 1.0)
@@ -193,6 +201,8 @@ class OCDriver:
     def step(self, x: torch.Tensor):
         xu = x.clamp(self.xmin, 1.0)
         rho = self.rho_min + (1.0 - self.rho_min) * xu ** self.penal
+        if self.start is not None:
+            self.start(rho, self.physics)
         CH, kap, iters = self.cells.homogenize(rho, self.physics, rtol=self.rtol, maxit=self.maxit)
         CHh = CH.cpu().numpy() if CH is not None else None
         kh = kap.cpu().numpy() if kap is not None else None

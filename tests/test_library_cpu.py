@@ -110,3 +110,26 @@ def test_workload_designs_match_oracle():
     from oracle import driver
     assert np.array_equal(optim.x0_sphere_void(16), driver.x0_sphere_void(16))
     assert np.array_equal(optim.x0_uniform_noise(16), driver.x0_uniform_noise(16))
+
+
+def test_warmstart_network_layout_matches_oracle(lib):
+    """Parameter count and initialisation of the warm-start network agree between the
+    library, its Python wrapper and the FP64 restatement; the layout helpers invert."""
+    import torch
+    from maple_amd import learned
+    from oracle import unet as ounet
+    assert lib.maple_unet_num_params() == learned.num_params() == ounet.num_params()
+    assert [l[0] for l in learned.LAYERS] == [l[0] for l in ounet.LAYERS]
+    np.testing.assert_array_equal(learned.init_params(3), ounet.init_params(3).astype(np.float32))
+    ue = torch.randn(6, 3, 64, dtype=torch.float64)
+    ut = torch.randn(3, 1, 64, dtype=torch.float64)
+    ch = learned.fields_to_channels(ue, ut)
+    assert ch.shape == (64, 21)
+    e, t = learned.channels_to_fields(ch)
+    np.testing.assert_allclose(e.numpy(), (ue - ue.mean(dim=2, keepdim=True)).numpy(), atol=1e-12)
+    np.testing.assert_allclose(t.numpy(), (ut - ut.mean(dim=2, keepdim=True)).numpy(), atol=1e-12)
+    # the feature of a cell twice the grid is the 2x2x2 mean of rho^(1/3)
+    rho = torch.rand(8 ** 3, dtype=torch.float64)
+    f = learned.density_feature(rho, 8, 4).numpy()
+    ref = (rho.numpy() ** (1 / 3)).reshape(4, 2, 4, 2, 4, 2).mean(axis=(1, 3, 5)).ravel()
+    np.testing.assert_allclose(f, ref, rtol=1e-6)
