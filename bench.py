@@ -12,11 +12,15 @@ disconnected islands slow the cell solves down.
 Forms (one JSON line; ``value`` is the headline form):
 * ``plain``    one cell, FP64, zero start every solve: the reference's method on the GPU.
 * ``headline`` 16 cells advanced together (one set of launches per OC step),
-               FP64 CG around an FP32 multigrid preconditioner, each solve
-               starting from the best of zero, the previous iterate's fields and
-               the learned network's prediction.  value = cells x iterations / s.
-* optional (--forms): ``fp64-warm``, ``mixed-warm`` (one cell), ``config4`` (the
-  64-cell mixed-resolution batch, 64^3-equivalent iterations / s).
+               block CG (the 6 or 3 load cases of a cell share one Krylov
+               space) in FP64 around an FP32 multigrid preconditioner, each
+               solve starting from the best of zero, the previous iterate's
+               fields and the learned network's prediction.
+               value = cells x iterations / s.
+* optional (--forms): single-cell and batched ablations (``fp64-warm``,
+  ``fp64-block``, ``mixed-warm``, ``mixed-warm-block``, ``mixed-warm-learned``,
+  ``batched-mixed-warm``, ``batched-mixed-warm-block``); --config4 adds the 64-cell
+  mixed-resolution batch (64^3-equivalent iterations / s).
 
 Accuracy: the last iterate's C_H and kappa of the headline form (every cell)
 against a tight (rtol 1e-11) FP64 solve of the same densities.  ``roofline`` is
@@ -52,7 +56,7 @@ def _x0(ncells):
 
 
 def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", warm: bool = False,
-             learned: bool = False, maxit: int = 5000, profile_kernel: bool = True):
+             learned: bool = False, block: bool = False, maxit: int = 5000, profile_kernel: bool = True):
     import numpy as np
     import torch
 
@@ -61,7 +65,8 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
     dev = torch.device("cuda", 0)
 
     def make():
-        cells = maple_amd.CellBatch([R] * ncells, nlevels=NLEVELS, device=dev, warm_start=warm, precision=precision)
+        cells = maple_amd.CellBatch([R] * ncells, nlevels=NLEVELS, device=dev, warm_start=warm, precision=precision,
+                                    block_cg=block)
         start = None
         if learned:
             from maple_amd import learned as ml
@@ -168,12 +173,15 @@ def cpu_baseline(n_iter: int):
 
 
 FORMS = {
-    "plain": dict(ncells=1, precision="fp64", warm=False, learned=False),
-    "headline": dict(ncells=16, precision="mixed", warm=True, learned=True),
-    "fp64-warm": dict(ncells=1, precision="fp64", warm=True, learned=False),
-    "mixed-warm": dict(ncells=1, precision="mixed", warm=True, learned=False),
-    "mixed-warm-learned": dict(ncells=1, precision="mixed", warm=True, learned=True),
-    "batched-mixed-warm": dict(ncells=16, precision="mixed", warm=True, learned=False),
+    "plain": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False),
+    "headline": dict(ncells=16, precision="mixed", warm=True, learned=True, block=True),
+    "fp64-warm": dict(ncells=1, precision="fp64", warm=True, learned=False, block=False),
+    "fp64-block": dict(ncells=1, precision="fp64", warm=False, learned=False, block=True),
+    "mixed-warm": dict(ncells=1, precision="mixed", warm=True, learned=False, block=False),
+    "mixed-warm-block": dict(ncells=1, precision="mixed", warm=True, learned=False, block=True),
+    "mixed-warm-learned": dict(ncells=1, precision="mixed", warm=True, learned=True, block=False),
+    "batched-mixed-warm": dict(ncells=16, precision="mixed", warm=True, learned=False, block=False),
+    "batched-mixed-warm-block": dict(ncells=16, precision="mixed", warm=True, learned=False, block=True),
 }
 
 
@@ -221,7 +229,8 @@ def main():
                                "x0=0.25+U(-0.05,0.05) (seed = cell index), J=0.5E+0.5kappa, vf 0.25, move 0.2, "
                                "filter k=3, 4-level MG-PCG rtol 1e-6",
                    "form": head, "cells": spec["ncells"], "precision": spec["precision"],
-                   "warm_start": spec["warm"], "learned_start": spec["learned"], "resolution": R,
+                   "warm_start": spec["warm"], "learned_start": spec["learned"], "block_cg": spec["block"],
+                   "resolution": R,
                    "levels": NLEVELS, "rtol": 1e-6},
         "ch_rel_err_vs_fp64_tight_solve": g["ch_rel_err"],
         "forms": {f: {k: res[f][k] for k in ("value", "ms_per_step", "cells", "ch_rel_err", "iters_el_mean",

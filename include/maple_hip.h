@@ -88,7 +88,14 @@ int maple_set_smoother(maple_ctx* ctx, int kind);
 
 /* CG variant: 0 = Fletcher-Reeves beta, 1 = flexible (Polak-Ribiere) beta, which
  * stays valid for a preconditioner that is not exactly symmetric (one extra dot
- * product per iteration), -1 = flexible in MAPLE_PREC_MIXED only (default). */
+ * product per iteration), -1 = flexible in MAPLE_PREC_MIXED only (default),
+ * 2 = block CG: the load cases of a cell share one Krylov space (search
+ * directions K-conjugate block to block, orthonormalised with dependent
+ * directions dropped), so a slow mode common to the cases is paid for once;
+ * a cell stops when all its cases meet the tolerance.  Block steps solve with
+ * P^T K P, whose conditioning follows the design's contrast, so after 200 block
+ * iterations per-case CG takes over from the block's fields.  The reference
+ * solves the cases one after another (model.py:111-124). */
 int maple_set_cg(maple_ctx* ctx, int flexible);
 
 /* MAPLE_PREC_MIXED only: a solve that has not converged after `iters` CG
@@ -214,9 +221,15 @@ int maple_unet_get_params(maple_unet* net, float* theta_host);
 /* feat_dev: batch*n^3 FP32 -> out_dev: batch*n^3*21 FP32 (raw network output). */
 int maple_unet_forward(maple_unet* net, const float* feat_dev, int batch, float* out_dev, void* stream);
 
-/* Loss mean((y - t)^2) over batch*n^3*21 values against BF16 targets (raw bits,
- * same layout as the output) and its gradient (FP32, parameter order) into
- * grad_dev (may be NULL).  Synchronizes the stream to read the loss. */
+/* Loss weighting: 0 = plain mean((y - t)^2); 1 = mean(w_v (y - t)^2) with
+ * w_v = f_v^3 + 1e-3 per voxel, f the (BF16-rounded) input feature, which is
+ * about the density under SIMP p = 3: the solver's residual sees a field error
+ * in proportion to the local stiffness. */
+int maple_unet_set_loss(maple_unet* net, int weighted);
+
+/* Loss (per maple_unet_set_loss) over batch*n^3*21 values against BF16 targets
+ * (raw bits, same layout as the output) and its gradient (FP32, parameter order)
+ * into grad_dev (may be NULL).  Synchronizes the stream to read the loss. */
 int maple_unet_loss_grad(maple_unet* net, const float* feat_dev, const uint16_t* target_dev, int batch,
                          double* loss_host, float* grad_dev, void* stream);
 

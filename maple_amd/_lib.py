@@ -67,6 +67,7 @@ SIGNATURES = {
     "maple_unet_destroy": (_I, [_P]),
     "maple_unet_set_params": (_I, [_P, C.POINTER(C.c_float)]),
     "maple_unet_get_params": (_I, [_P, C.POINTER(C.c_float)]),
+    "maple_unet_set_loss": (_I, [_P, _I]),
     "maple_unet_forward": (_I, [_P, _P, _I, _P, _P]),
     "maple_unet_loss_grad": (_I, [_P, _P, _P, _I, _DP, _P, _P]),
     "maple_unet_train_step": (_I, [_P, _P, _P, _I, C.c_float, _DP, _P]),

@@ -43,7 +43,7 @@ def max_levels(r: int, min_coarse: int = 4) -> int:
 class CellBatch:
     def __init__(self, dims: Sequence, nlevels: int = 4, device=None, precision: str = "fp64",
                  omega: Optional[float] = None, nsmooth: int = 2, warm_start: bool = False,
-                 smoother: str = "l1-jacobi"):
+                 smoother: str = "l1-jacobi", block_cg: bool = False):
         self.lib = L.load()
         dl = []
         for d in dims:
@@ -74,6 +74,8 @@ class CellBatch:
         if omega is None:
             omega = 1.6 if smoother == "l1-jacobi" else 0.6
         self.set_solver(omega, nsmooth, warm_start)
+        if block_cg:
+            self.set_cg(block=True)
 
     # -- lifetime ---------------------------------------------------------
     def close(self):
@@ -113,9 +115,11 @@ class CellBatch:
         k = {"jacobi": L.MAPLE_SMOOTH_JACOBI, "l1-jacobi": L.MAPLE_SMOOTH_L1_JACOBI}[kind]
         L.check(self.lib.maple_set_smoother(self._h, k))
 
-    def set_cg(self, flexible: Optional[bool] = None):
-        """Flexible (Polak-Ribiere) CG on/off; None = only in the mixed form."""
-        L.check(self.lib.maple_set_cg(self._h, -1 if flexible is None else int(bool(flexible))))
+    def set_cg(self, flexible: Optional[bool] = None, block: bool = False):
+        """Flexible (Polak-Ribiere) CG on/off (None = only in the mixed form), or block CG
+        over the load cases of each cell."""
+        mode = 2 if block else (-1 if flexible is None else int(bool(flexible)))
+        L.check(self.lib.maple_set_cg(self._h, mode))
 
     def set_mixed_fallback(self, iters: int = 100):
         """Mixed form: finish a solve on the FP64 V-cycle once it has run `iters` CG

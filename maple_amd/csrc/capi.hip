@@ -82,6 +82,7 @@ struct Phys {
   double* atol2 = nullptr;
   bool have_fields = false, have_energy = false;
   double* ustart = nullptr;   // supplied start of the next solve
+  BlockScalars* bs = nullptr; // block CG state, one per cell
   bool have_start = false;
   long long setups = 0;    // hierarchy builds so far (drives the coarsest-inverse refresh)
 };
@@ -103,7 +104,8 @@ struct maple_ctx {
   int nsmooth = 2;
   int warm = 0;
   int smoother = MAPLE_SMOOTH_L1_JACOBI;
-  int flexible = -1;   // -1: flexible CG in the mixed form only
+  int flexible = -1;   // -1: flexible CG in the mixed form only; 2: block CG
+  int block_max = 200; // block CG iterations before per-case CG takes over
   int coarse_refresh = 1;   // rebuild the coarsest inverse every this many homogenizations
   int fallback = 100;       // mixed form: FP64 V-cycle after this many CG iterations (0: never)
   OCState* ocst = nullptr;
@@ -224,6 +226,7 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
   TRY(dalloc(c, &P.q, n0));
   TRY(dalloc(c, &P.f, n0));
   TRY(dalloc(c, &P.ustart, n0));
+  TRY(dalloc(c, &P.bs, (size_t)c->ncells));
   HIPCHK(hipMemset(P.u, 0, n0 * sizeof(double)));
   // dense coarsest level
   const Level& LC = c->lv[L - 1];
@@ -483,6 +486,87 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   return all ? 0 : MAPLE_ENOCONV;
 }
 
+// Block CG: the cases of a cell share one Krylov space (bcg.hip).
+int bpcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStream_t s) {
+  Level& L0 = c->lv[0];
+  const Launch& LN = P.ln[0];
+  bool use64 = c->prec == MAPLE_PREC_FP64;
+  vec_zero<int>((long long)c->ncells * P.nc, P.done_d, s);
+  fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
+  vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, c->partial, s);
+  reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_FF, s);
+  const bool have_prev = c->warm && P.have_fields;
+  if (have_prev) {
+    apply_fine64(c, P, 2, rho, P.u, P.f, P.r, nullptr, s);
+    vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, c->partial, s);
+    reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
+  }
+  if (P.have_start) {
+    apply_fine64(c, P, 2, rho, P.ustart, P.f, P.q, nullptr, s);
+    vec_dot<double, double>(L0.node, P.nd, P.nc, P.q, P.q, c->partial, s);
+    reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RRS, s);
+  }
+  pcg_pick_start(L0.node, c->ncells, P.nd, P.nc, have_prev, P.have_start, P.sc, P.u, P.r, P.ustart, P.q, P.f, s);
+  P.have_start = false;
+  vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, c->partial, s);
+  reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
+  pcg_init_scalars(c->ncells, P.nc, rtol, P.atol2, P.sc, P.done_d, s);
+  bcg_init(c->ncells, P.nc, P.bs, P.sc, P.done_d, s);
+  double* z = precondition(c, P, P.r, use64, s);
+  bcg_gram(LN, P.nd, P.nc, 1, z, z, nullptr, c->partial, P.bs, s);   // translation content of Z
+  bcg_new_direction(LN, P.nd, P.nc, true, P.bs, z, P.p, c->partial, s);
+  TRY(enqueue_poll(c, P, 0, s));
+  bool all = false;
+  TRY(poll_done(c, P, 0, &all));
+  for (int it = 0; it < maxit && !all; ++it) {
+    apply_fine64(c, P, 1, rho, P.p, nullptr, P.q, c->partial, s);   // Q = K P (live cells only)
+    bcg_gram(LN, P.nd, P.nc, 2, P.p, P.q, P.r, c->partial, P.bs, s);
+    bcg_alpha(c->ncells, P.nc, P.bs, s);
+    bcg_update_xr(LN, P.nd, P.nc, P.bs, P.p, P.q, P.u, P.r, c->partial, s);
+    reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
+    bcg_check(c->ncells, P.nc, P.bs, P.sc, P.done_d, s);
+    const int slot = (it + 1) & 1;
+    TRY(enqueue_poll(c, P, slot, s));
+    if (it + 1 < maxit) {
+      if (!use64 && c->fallback > 0 && it + 1 == c->fallback) {
+        setup_fp64(c, P, rho, s, true);
+        use64 = true;
+      }
+      z = precondition(c, P, P.r, use64, s);
+      bcg_gram(LN, P.nd, P.nc, 1, z, P.q, nullptr, c->partial, P.bs, s);
+      bcg_beta(c->ncells, P.nc, P.bs, s);
+      bcg_new_direction(LN, P.nd, P.nc, false, P.bs, z, P.p, c->partial, s);
+    }
+    TRY(poll_done(c, P, slot, &all));
+  }
+  P.have_fields = true;
+  HIPCHK(hipMemcpyAsync(P.hsc.data(), P.sc, sizeof(CaseScalars) * P.hsc.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (const CaseScalars& cs : P.hsc)
+    if (!(cs.rr == cs.rr) || !std::isfinite(cs.ff)) return fail(MAPLE_ENONFINITE, "non-finite residual in the cell solve");
+  return all ? 0 : MAPLE_ENOCONV;
+}
+
+// The cell solve of one physics: per-case CG, or block CG that hands over to
+// per-case CG (started from the block's fields) after block_max iterations.
+// Block steps solve with the Gram matrix P^T K P, whose condition grows with the
+// design's contrast; far below rtol 1e-7 a block can stall where per-case CG
+// still converges.
+int solve_cases(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStream_t s) {
+  if (c->flexible != 2) return pcg(c, P, rho, rtol, maxit, s);
+  const int nb = std::min(maxit, c->block_max);
+  int rc = bpcg(c, P, rho, rtol, nb, s);
+  if (rc != MAPLE_ENOCONV || nb >= maxit) return rc;
+  std::vector<int> it0(P.hsc.size());
+  for (size_t i = 0; i < P.hsc.size(); ++i) it0[i] = P.hsc[i].iters;
+  const int warm = c->warm;
+  c->warm = 1;
+  rc = pcg(c, P, rho, rtol, maxit - nb, s);
+  c->warm = warm;
+  for (size_t i = 0; i < P.hsc.size(); ++i) P.hsc[i].iters += it0[i];
+  return rc;
+}
+
 int check_ctx(const maple_ctx* c) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
   return set_device(c);
@@ -574,9 +658,9 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
       if ((rc = make_launch(c, L, 1024, false, &L.doblk, &L.oc))) return bail(rc);
     }
   }
-  // partial-sum scratch: 64 doubles per block of the largest launch
+  // partial-sum scratch: 80 doubles per block of the largest launch (block CG: two 6x6 Grams)
   int maxb = std::max(c->lv[0].node.nblocks, c->lv[0].fine.nblocks);
-  if ((rc = dalloc(c, &c->partial, (size_t)maxb * 64))) return bail(rc);
+  if ((rc = dalloc(c, &c->partial, (size_t)maxb * 80))) return bail(rc);
   if ((rc = alloc_phys(c, c->ph[0], 3, 6))) return bail(rc);
   if ((rc = alloc_phys(c, c->ph[1], 1, 3))) return bail(rc);
   for (int p = 0; p < 2; ++p) {
@@ -676,7 +760,7 @@ int maple_set_mixed_fallback(maple_ctx* c, int iters) {
 
 int maple_set_cg(maple_ctx* c, int flexible) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
-  if (flexible < -1 || flexible > 1) return fail(MAPLE_EINVAL, "flexible must be -1, 0 or 1");
+  if (flexible < -1 || flexible > 2) return fail(MAPLE_EINVAL, "CG variant must be -1, 0, 1 or 2");
   c->flexible = flexible;
   return 0;
 }
@@ -702,7 +786,7 @@ int maple_homogenize(maple_ctx* c, const double* rho, int physics, double rtol, 
     double* out = p == 0 ? CH : kappa;
     setup_hierarchy(c, P, rho, s);
     HIPCHK(hipGetLastError());
-    int rc = pcg(c, P, rho, rtol, maxit, s);
+    int rc = solve_cases(c, P, rho, rtol, maxit, s);
     if (rc == MAPLE_ENOCONV) {
       status = rc;
       g_err = std::string(p == 0 ? "elastic" : "thermal") + " CG reached maxit before rtol";

@@ -96,6 +96,34 @@ struct CaseScalars {   // one per (cell, case)
   int src;             // start of the last solve: 0 zero, 1 previous fields, 2 supplied start
   int pad_;
 };
+// Block CG state of one cell (bcg.hip): Gram matrices of the current step (two
+// NC x NC, row-major, at 0 and 36), the step matrices, the Cholesky factor of
+// P^T K P, the orthonormalising transform and the active directions.
+struct BlockScalars {
+  double gram[72];
+  double alpha[36], beta[36], T[36], lg[36];
+  int active[MAPLE_MAX_CASES];
+  int done, pad_;
+};
+// ng = 2: A^T B0, A^T B1 into bs[].gram[0..36), [36..72); ng = 1: A^T B0 and the
+// per-(case, component) sums of A (its translation content) at gram[36..)
+void bcg_gram(const Launch& L, int nd, int nc, int ng, const double* A, const double* B0, const double* B1,
+              double* partial, BlockScalars* bs, hipStream_t s);
+// alpha = (P^T Q)^-1 P^T R from gram = (P^T Q, P^T R)
+void bcg_alpha(int ncells, int nc, BlockScalars* bs, hipStream_t s);
+// X += P alpha; R -= Q alpha; partial[blk*6 + j] = (r_j, r_j)
+void bcg_update_xr(const Launch& L, int nd, int nc, const BlockScalars* bs, const double* P, const double* Q,
+                   double* X, double* R, double* partial, hipStream_t s);
+// per cell: done when every case meets its target; iteration counts
+void bcg_check(int ncells, int nc, BlockScalars* bs, CaseScalars* sc, int* done_flags, hipStream_t s);
+// beta = -(P^T Q)^-1 Q^T Z from gram = Z^T Q
+void bcg_beta(int ncells, int nc, BlockScalars* bs, hipStream_t s);
+// P = orth(Z - translations + P beta) (first: no P term); the translation sums of Z
+// must be in bs[].gram[36..) (bcg_gram with ng = 1)
+void bcg_new_direction(const Launch& L, int nd, int nc, bool first, BlockScalars* bs, const double* Z, double* P,
+                       double* partial, hipStream_t s);
+void bcg_init(int ncells, int nc, BlockScalars* bs, CaseScalars* sc, int* done_flags, hipStream_t s);
+
 // sum partial[blk*6 + j] over each cell's blocks into dst (a member of CaseScalars)
 void reduce_partials(const int* blk_start, int ncells, int ncase, const double* partial,
                      CaseScalars* sc, int which, hipStream_t s);

@@ -309,18 +309,26 @@ __global__ void k_relu_grad(const float* __restrict__ d, const u16* __restrict__
   if (t < n) g[t] = bf2f(act[t]) > 0.f ? f2bf(d[t]) : (u16)0;
 }
 
-// Mean-squared error over (nvox, 21); gradient 2 (y - t) / size into the padded
-// head gradient, per-block loss partials.
-__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const u16* __restrict__ tgt, int nvox,
-                                             float inv_size, u16* __restrict__ g, float* __restrict__ partial) {
+// Mean-squared error over (nvox, 21), each voxel weighted by w_v (1, or
+// f_v^3 + 1e-3 with f the network input, about the element density: the solver's
+// residual sees a field error in proportion to the local stiffness); gradient
+// 2 w_v (y - t) / size into the padded head gradient, per-block loss partials.
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const u16* __restrict__ tgt,
+                                             const u16* __restrict__ feat, int weighted, int nvox, float inv_size,
+                                             u16* __restrict__ g, float* __restrict__ partial) {
   __shared__ float red[256];
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   float s = 0.f;
   if (v < nvox) {
+    float w = 1.f;
+    if (weighted) {
+      const float f = bf2f(feat[v]);
+      w = f * f * f + 1e-3f;
+    }
     for (int c = 0; c < NOUT; ++c) {
       const float d = y[(size_t)v * NOUT + c] - bf2f(tgt[(size_t)v * NOUT + c]);
-      s += d * d;
-      g[(size_t)v * GHEAD_LD + c] = f2bf(2.f * d * inv_size);
+      s += w * d * d;
+      g[(size_t)v * GHEAD_LD + c] = f2bf(2.f * w * d * inv_size);
     }
     for (int c = NOUT; c < GHEAD_LD; ++c) g[(size_t)v * GHEAD_LD + c] = 0;
   }
@@ -408,6 +416,7 @@ struct Layer {
 
 struct maple_unet {
   int device = 0, n = 0, max_batch = 0;
+  int weighted = 0;   // loss weight f^3 + 1e-3 per voxel (f: input feature)
   std::vector<Layer> L;
   long long nparam = 0;
   float *theta = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
@@ -535,7 +544,8 @@ int backward(maple_unet* u, const u16* target, int batch, double* loss, hipStrea
   const int n0 = nvox_at(u, 0, batch);
   const int nblk = cdiv(n0, 256);
   u16* ghead = u->gbf[nl - 1];
-  hipLaunchKernelGGL(k_mse, dim3(nblk), dim3(256), 0, s, u->out, target, n0, 1.f / ((float)n0 * NOUT), ghead, u->lpart);
+  hipLaunchKernelGGL(k_mse, dim3(nblk), dim3(256), 0, s, u->out, target, u->feat, u->weighted, n0,
+                     1.f / ((float)n0 * NOUT), ghead, u->lpart);
   for (int li = 0; li < nl - 1; ++li)
     fill0(u->dact[li], (long long)nvox_at(u, u->L[li].level, batch) * u->L[li].cout, s);
   fill0(u->dpool1, (long long)nvox_at(u, 1, batch) * 16, s);
@@ -726,6 +736,13 @@ int maple_unet_get_params(maple_unet* u, float* theta_host) {
   if (hipSetDevice(u->device) != hipSuccess) return ufail(MAPLE_EDEVICE, "hipSetDevice failed");
   UCHK(hipDeviceSynchronize());
   UCHK(hipMemcpy(theta_host, u->theta, sizeof(float) * u->nparam, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int maple_unet_set_loss(maple_unet* u, int weighted) {
+  if (!u) return ufail(MAPLE_EINVAL, "null network");
+  if (weighted != 0 && weighted != 1) return ufail(MAPLE_EINVAL, "weighted must be 0 or 1");
+  u->weighted = weighted;
   return 0;
 }
 

@@ -78,6 +78,7 @@ class UNet:
         self.nparam = int(self.lib.maple_unet_num_params())
         self.set_params(init_params(seed) if theta is None else theta)
         self.scales = np.ones(NCH, dtype=np.float32) if scales is None else np.asarray(scales, dtype=np.float32)
+        self.weighted = False
 
     def close(self):
         if getattr(self, "_h", None):
@@ -98,6 +99,11 @@ class UNet:
         if th.size != self.nparam:
             raise ValueError(f"expected {self.nparam} parameters, got {th.size}")
         L.check(self.lib.maple_unet_set_params(self._h, th.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def set_loss(self, weighted: bool):
+        """Voxel weight f^3 + 1e-3 in the training loss (f: input feature) or none."""
+        L.check(self.lib.maple_unet_set_loss(self._h, int(bool(weighted))))
+        self.weighted = bool(weighted)
 
     def params(self) -> np.ndarray:
         th = np.empty(self.nparam, dtype=np.float32)
@@ -142,7 +148,7 @@ class UNet:
 
     def save(self, path: str, **meta):
         os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-        np.savez(path, theta=self.params(), scales=self.scales, n=np.int64(self.n),
+        np.savez(path, theta=self.params(), scales=self.scales, n=np.int64(self.n), weighted=np.int64(self.weighted),
                  **{k: np.asarray(v) for k, v in meta.items()})
 
     @classmethod
@@ -321,13 +327,24 @@ class LearnedStart:
 
     @torch.no_grad()
     def __call__(self, rho: torch.Tensor, physics: Sequence[str] = ("elastic", "thermal")):
-        cells, n = self.cells, self.net.n
         if not self.l0 and not self.l1:
             return
+        el0, th0 = self.predict(rho)
+        if "elastic" in physics:
+            self.cells.set_start("elastic", el0)
+        if "thermal" in physics:
+            self.cells.set_start("thermal", th0)
+
+    @torch.no_grad()
+    def predict(self, rho: torch.Tensor):
+        """Predicted level-0 fields ((6, 3, Etot), (3, 1, Etot)); zero for cells the network does not cover."""
+        cells, n = self.cells, self.net.n
         dev = cells.device
         el0 = torch.zeros(6, 3, cells.nelem, dtype=torch.float64, device=dev)
         th0 = torch.zeros(3, 1, cells.nelem, dtype=torch.float64, device=dev)
         idx = self.l0 + self.l1
+        if not idx:
+            return el0, th0
         feats = []
         for ci in idx:
             r = cells.dims[ci][0]
@@ -351,7 +368,4 @@ class LearnedStart:
         if self.l1:
             el0 += cells.prolong("elastic", 1, el1)
             th0 += cells.prolong("thermal", 1, th1)
-        if "elastic" in physics:
-            cells.set_start("elastic", el0)
-        if "thermal" in physics:
-            cells.set_start("thermal", th0)
+        return el0, th0

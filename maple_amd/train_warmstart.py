@@ -41,20 +41,31 @@ def measure_gain(net: learned.UNet, dev, res: int = 64, iters: int = 60, probe_e
             rho = info["rho"]
             _, _, cold = probe.homogenize(rho, rtol=1e-6, maxit=5000)
             cold = cold[0].copy()
+            # residual of the predicted start relative to the zero start's, per case
+            ratio = []
+            for phys, u in zip(("elastic", "thermal"), start.predict(rho)):
+                f = probe.rhs(phys, rho)
+                r = f - probe.apply_operator(phys, rho, u)
+                ratio += (torch.linalg.norm(r.reshape(r.shape[0], -1), dim=1)
+                          / torch.linalg.norm(f.reshape(f.shape[0], -1), dim=1)).tolist()
             start(rho)
             _, _, warm = probe.homogenize(rho, rtol=1e-6, maxit=5000)
             warm = warm[0].copy()
             src = np.concatenate([probe.start_sources("elastic")[0], probe.start_sources("thermal")[0]])
-            rows.append({"oc_iter": it, "cold": cold.tolist(), "learned": warm.tolist(), "src": src.tolist()})
+            rows.append({"oc_iter": it, "cold": cold.tolist(), "learned": warm.tolist(), "src": src.tolist(),
+                         "start_residual_ratio": ratio})
             log(f"OC {it}: cold max el {cold[:6].max()} th {cold[6:].max()}  learned max el {warm[:6].max()} "
-                f"th {warm[6:].max()}  starts {src.tolist()}", flush=True)
+                f"th {warm[6:].max()}  starts {src.tolist()}  start residual / zero-start residual "
+                f"{np.round(ratio, 3).tolist()}", flush=True)
         x = x_new
     cold_tot = sum(sum(r["cold"]) for r in rows)
     warm_tot = sum(sum(r["learned"]) for r in rows)
     cells.close()
     probe.close()
+    ratios = np.array([r["start_residual_ratio"] for r in rows])
     return {"probes": rows, "cold_total_iters": cold_tot, "learned_total_iters": warm_tot,
-            "iteration_ratio": warm_tot / max(1, cold_tot)}
+            "iteration_ratio": warm_tot / max(1, cold_tot),
+            "median_start_residual_ratio": float(np.median(ratios))}
 
 
 def main():
@@ -67,6 +78,7 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--gain-iters", type=int, default=60)
+    ap.add_argument("--unweighted", action="store_true", help="plain MSE instead of the density-weighted loss")
     args = ap.parse_args()
     dev = torch.device("cuda", 0)
     t0 = time.time()
@@ -75,6 +87,7 @@ def main():
     scales = learned.channel_scales(tgt)
     print(f"{feat.shape[0]} pairs in {t_data:.1f} s; channel scales {np.round(scales, 5).tolist()}", flush=True)
     net = learned.UNet(n=32, max_batch=args.batch, device=dev, seed=0, scales=scales)
+    net.set_loss(not args.unweighted)
     t1 = time.time()
     hist = learned.train(net, feat, tgt, epochs=args.epochs, batch=args.batch, lr=args.lr)
     t_train = time.time() - t1
@@ -87,7 +100,8 @@ def main():
     print(json.dumps({"pairs": args.cells * args.iters, "data_s": t_data, "train_s": t_train,
                       "final_train_loss": hist[-1][1], "final_val_loss": hist[-1][2],
                       "cold_total_iters": gain["cold_total_iters"], "learned_total_iters": gain["learned_total_iters"],
-                      "iteration_ratio": gain["iteration_ratio"], "out": args.out}), flush=True)
+                      "iteration_ratio": gain["iteration_ratio"],
+                      "median_start_residual_ratio": gain["median_start_residual_ratio"], "out": args.out}), flush=True)
 
 
 if __name__ == "__main__":

@@ -165,18 +165,27 @@ def forward(theta, feat, q=None):
     return acts["head"], (acts, inputs)
 
 
-def loss(theta, feat, target, q=None):
+def voxel_weight(feat, q=None, weighted=False):
+    """Per-voxel loss weight: 1, or f^3 + 1e-3 of the (rounded) input feature."""
+    if not weighted:
+        return np.ones(feat.shape + (1,))
+    f = (q or _id)(feat)
+    return (f ** 3 + 1e-3)[..., None]
+
+
+def loss(theta, feat, target, q=None, weighted=False):
     y, _ = forward(theta, feat, q)
-    return float(np.mean((y - target) ** 2))
+    return float(np.mean(voxel_weight(feat, q, weighted) * (y - target) ** 2))
 
 
-def backward(theta, feat, target, q=None):
-    """Loss mean((y - t)^2) and its gradient with respect to the flat parameters."""
+def backward(theta, feat, target, q=None, weighted=False):
+    """Loss mean(w (y - t)^2) and its gradient with respect to the flat parameters."""
+    w = voxel_weight(feat, q, weighted)
     q = q or _id
     p = unflatten(theta)
     y, (acts, inputs) = forward(theta, feat, q)
-    L = float(np.mean((y - target) ** 2))
-    dact = {"head": 2.0 * (y - target) / y.size}
+    L = float(np.mean(w * (y - target) ** 2))
+    dact = {"head": 2.0 * w * (y - target) / y.size}
     grads = {}
     for name, srcs, _, _, _, relu, _ in reversed(LAYERS):
         g = dact.pop(name)

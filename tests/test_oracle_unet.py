@@ -1,7 +1,7 @@
 """Pins the FP64 warm-start network restatement: periodic convolution against
 scipy.ndimage.correlate (mode "wrap"), pooling and upsampling against explicit
 loops, every gradient against central finite differences, bf16 rounding against
-exhaustive float32 rounding rules."""
+the float32 bit pattern rounding rule."""
 import numpy as np
 import pytest
 from scipy import ndimage
@@ -66,21 +66,22 @@ def test_conv_backward_finite_differences():
     assert np.allclose(db, g.sum(axis=(0, 1, 2, 3)))
 
 
-def test_network_gradient_finite_differences():
+@pytest.mark.parametrize("weighted", [False, True])
+def test_network_gradient_finite_differences(weighted):
     theta = unet.init_params(3)
     assert theta.size == unet.num_params()
     rng = np.random.default_rng(4)
     feat = rng.uniform(0, 1, (2, 4, 4, 4))
     target = rng.standard_normal((2, 4, 4, 4, unet.NOUT))
-    L, g = unet.backward(theta, feat, target)
-    assert np.isclose(L, unet.loss(theta, feat, target))
+    L, g = unet.backward(theta, feat, target, weighted=weighted)
+    assert np.isclose(L, unet.loss(theta, feat, target, weighted=weighted))
     h = 1e-6
     idx = rng.choice(theta.size, 40, replace=False)
-    # include every bias of the first and the last layer
     for k in idx:
         e = np.zeros_like(theta)
         e[k] = h
-        fd = (unet.loss(theta + e, feat, target) - unet.loss(theta - e, feat, target)) / (2 * h)
+        fd = (unet.loss(theta + e, feat, target, weighted=weighted)
+              - unet.loss(theta - e, feat, target, weighted=weighted)) / (2 * h)
         assert abs(fd - g[k]) <= 1e-5 * max(1e-3, abs(g[k])) + 1e-9, (k, fd, g[k])
 
 
