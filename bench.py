@@ -46,6 +46,10 @@ NLEVELS = 4
 OBJ = "stiffness:0.5,thermal:0.5"
 PEAK = {"fp64": 78.6, "mixed": 157.3}
 FLOP_PER_ELEM_ELASTIC = 1152      # 24 rows x 24 cols x 2 per element and load case
+# HBM bytes per 64^3 cell and launch of the level-0 elastic operator, from rocprofv3
+# FETCH_SIZE + WRITE_SIZE runs calibrated against a copy kernel
+# (profiles/fine_operator_64cube_hbm_counters.md): FP64 product 87.6 MB, FP32 Jacobi sweep 57.1 MB
+TRAFFIC_PER_CELL = {"fp64": 87.6e6, "mixed": 57.1e6}
 
 
 def _x0(ncells):
@@ -126,7 +130,8 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
         tname = "double" if precision == "fp64" else "float"
         out["kernel"] = {"name": f"k_fine_apply<3,{tname},*> (level-0 elastic operator, 6 load cases x {ncells} cells)",
                          "launches": count, "avg_us": avg_s * 1e6, "flop_per_launch": flops,
-                         "achieved_tflops": flops / avg_s / 1e12, "peak": PEAK[precision]}
+                         "achieved_tflops": flops / avg_s / 1e12, "peak": PEAK[precision],
+                         "traffic": ncells * TRAFFIC_PER_CELL[precision]}
     return out
 
 
@@ -242,7 +247,8 @@ def main():
     k = g.get("kernel")
     if k:
         line["roofline"] = {"bound": "mfma", "achieved": k["achieved_tflops"], "peak": k["peak"],
-                            "unit": "TFLOP/s", "frac": k["achieved_tflops"] / k["peak"], "traffic": None,
+                            "unit": "TFLOP/s", "frac": k["achieved_tflops"] / k["peak"], "traffic": k["traffic"],
+                            "traffic_unit": "bytes per launch (rocprofv3 PMC, copy-calibrated)",
                             "kernel": k["name"], "avg_us": k["avg_us"], "launches": k["launches"],
                             "algorithmic_flop_per_launch": k["flop_per_launch"],
                             "roof": "vector FMA and MFMA share the MI355X peak for this precision "
