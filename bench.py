@@ -12,15 +12,15 @@ disconnected islands slow the cell solves down.
 Forms (one JSON line; ``value`` is the headline form):
 * ``plain``    one cell, FP64, zero start every solve: the reference's method on the GPU.
 * ``headline`` 16 cells advanced together (one set of launches per OC step),
-               block CG (the 6 or 3 load cases of a cell share one Krylov
-               space) in FP64 around an FP32 multigrid preconditioner, each
-               solve starting from the best of zero, the previous iterate's
-               fields and the learned network's prediction.
+               FP64 CG around an FP32 multigrid preconditioner, each solve
+               starting from the best of zero, the previous iterate's fields
+               and the learned network's prediction.
                value = cells x iterations / s.
 * optional (--forms): single-cell and batched ablations (``fp64-warm``,
   ``fp64-block``, ``mixed-warm``, ``mixed-warm-block``, ``mixed-warm-learned``,
-  ``batched-mixed-warm``, ``batched-mixed-warm-block``); --config4 adds the 64-cell
-  mixed-resolution batch (64^3-equivalent iterations / s).
+  ``batched-mixed-warm``, ``batched-mixed-warm-block``, ``headline-block`` with
+  block CG over the load cases); --config4 adds the 64-cell mixed-resolution
+  batch (64^3-equivalent iterations / s).
 
 Accuracy: the last iterate's C_H and kappa of the headline form (every cell)
 against a tight (rtol 1e-11) FP64 solve of the same densities.  ``roofline`` is
@@ -179,7 +179,8 @@ def cpu_baseline(n_iter: int):
 
 FORMS = {
     "plain": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False),
-    "headline": dict(ncells=16, precision="mixed", warm=True, learned=True, block=True),
+    "headline": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False),
+    "headline-block": dict(ncells=16, precision="mixed", warm=True, learned=True, block=True),
     "fp64-warm": dict(ncells=1, precision="fp64", warm=True, learned=False, block=False),
     "fp64-block": dict(ncells=1, precision="fp64", warm=False, learned=False, block=True),
     "mixed-warm": dict(ncells=1, precision="mixed", warm=True, learned=False, block=False),
