@@ -145,6 +145,13 @@ int maple_sensitivity(maple_ctx* ctx, const double* G6_dev, const double* G3_dev
  * main_IHD_v3.py:342-359).  in and out: Etot doubles, may not alias. */
 int maple_filter_box(maple_ctx* ctx, const double* in_dev, double* out_dev, int k, void* stream);
 
+/* Symmetry projection of an element field (symmetrize_vec, main_IHD_v3.py:221-337):
+ * kind 0 none (copy), 1 mirror about the centre planes of the axes in the bit
+ * mask `axes` (1 x, 2 y, 4 z), 2 the 48 symmetries of the cube (cubic cells).
+ * in and out: Etot doubles, may not alias.  The projection is self-adjoint, so
+ * it also maps dJ/dx_sym to dJ/dx. */
+int maple_symmetrize(maple_ctx* ctx, int kind, int axes, const double* in_dev, double* out_dev, void* stream);
+
 /* OC update with nbisect bisection steps on lambda (oc_update,
  * main_IHD_v3.py:362-380): volfrac/move/xmin are per-cell host arrays. */
 int maple_oc_update(maple_ctx* ctx, const double* x_dev, const double* dJdx_dev, const double* volfrac_host,

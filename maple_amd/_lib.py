@@ -50,6 +50,7 @@ SIGNATURES = {
     "maple_homogenize": (_I, [_P, _P, _I, _D, _I, _P, _P, _IP, _P]),
     "maple_sensitivity": (_I, [_P, _P, _P, _P, _P]),
     "maple_filter_box": (_I, [_P, _P, _P, _I, _P]),
+    "maple_symmetrize": (_I, [_P, _I, _I, _P, _P, _P]),
     "maple_oc_update": (_I, [_P, _P, _P, _DP, _DP, _DP, _I, _P, _P]),
     "maple_rhs": (_I, [_P, _I, _P, _P, _P]),
     "maple_apply_operator": (_I, [_P, _I, _P, _I, _P, _P, _P]),

@@ -197,6 +197,17 @@ class CellBatch:
                                          self._stream()))
         return out
 
+    def symmetrize(self, v: torch.Tensor, kind: str = "none", axes: Sequence[int] = ()) -> torch.Tensor:
+        """Mean over reflections about the centre planes of `axes` ("mirror") or the 48
+        cube symmetries ("cubic") of every cell."""
+        v = self._vec(v, self.nelem, "v")
+        out = torch.empty_like(v)
+        k = {"none": 0, "mirror": 1, "cubic": 2}[kind]
+        mask = sum(1 << int(a) for a in axes)
+        L.check(self.lib.maple_symmetrize(self._h, k, mask, C.c_void_p(v.data_ptr()), C.c_void_p(out.data_ptr()),
+                                          self._stream()))
+        return out
+
     def rhs(self, physics: str, rho: torch.Tensor) -> torch.Tensor:
         rho = self._vec(rho, self.nelem, "rho")
         F = self._empty(NCASE[physics], NDOF[physics], self.nelem)

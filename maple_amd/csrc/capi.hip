@@ -824,6 +824,25 @@ int maple_filter_box(maple_ctx* c, const double* in, double* out, int k, void* s
   return 0;
 }
 
+int maple_symmetrize(maple_ctx* c, int kind, int axes, const double* in, double* out, void* stream) {
+  TRY(check_ctx(c));
+  if (!in || !out || in == out) return fail(MAPLE_EINVAL, "need distinct input and output");
+  if (kind < 0 || kind > 2) return fail(MAPLE_EINVAL, "symmetry kind must be 0 (none), 1 (mirror) or 2 (cubic)");
+  if (kind == 1 && (axes < 0 || axes > 7)) return fail(MAPLE_EINVAL, "mirror axes must be a bit mask in [0, 7]");
+  if (kind == 2)
+    for (int i = 0; i < c->ncells; ++i)
+      if (c->dims[3 * i] != c->dims[3 * i + 1] || c->dims[3 * i] != c->dims[3 * i + 2])
+        return fail(MAPLE_EINVAL, "cubic symmetry needs cubic cells");
+  hipStream_t s = (hipStream_t)stream;
+  if (kind == 0 || (kind == 1 && axes == 0)) {
+    HIPCHK(hipMemcpyAsync(out, in, sizeof(double) * c->lv[0].Ntot, hipMemcpyDeviceToDevice, s));
+    return 0;
+  }
+  symmetrize(c->lv[0].node, kind, axes, in, out, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int maple_oc_update(maple_ctx* c, const double* x, const double* dJ, const double* volfrac, const double* move,
                     const double* xmin, int nbisect, double* xnew, void* stream) {
   TRY(check_ctx(c));

@@ -179,6 +179,51 @@ void box_filter(const Launch& L, int k, const double* in, double* out, hipStream
   k_box_filter<<<L.nblocks, 256, 0, s>>>(L, k, in, out);
 }
 
+// Symmetry projection of a design field (parse_symmetry / symmetrize_vec,
+// main_IHD_v3.py:221-337): the mean of the field over a group of index maps of
+// the cell.  kind 1: reflections about the centre planes of the axes in `axes`
+// (bit 0 x, bit 1 y, bit 2 z; 2^k images, x -> n-1-x); kind 2: the 48 symmetries
+// of the cube (6 axis permutations x 8 reflections; cubic cells).  Summing over
+// the whole group makes the result independent of how its elements are listed.
+__global__ __launch_bounds__(256) void k_symmetrize(Launch L, int kind, int axes, const double* __restrict__ in,
+                                                    double* __restrict__ out) {
+  const int ci = find_cell(L.blk_start, L.ncells, lbid());
+  const LevelCell c = L.cells[ci];
+  const long long n = (long long)(lbid() - L.blk_start[ci]) * 256 + threadIdx.x;
+  if (n >= (long long)c.nx * c.ny * c.nz) return;
+  const int z = (int)(n % c.nz), y = (int)((n / c.nz) % c.ny), x = (int)(n / ((long long)c.nz * c.ny));
+  const int dims[3] = {c.nx, c.ny, c.nz};
+  const int q[3] = {x, y, z};
+  double s = 0.0;
+  int cnt = 0;
+  if (kind == 1) {
+    for (int m = 0; m < 8; ++m) {
+      if (m & ~axes) continue;
+      int v[3];
+      for (int a = 0; a < 3; ++a) v[a] = (m >> a & 1) ? dims[a] - 1 - q[a] : q[a];
+      s += in[c.off + ((long long)v[0] * c.ny + v[1]) * c.nz + v[2]];
+      ++cnt;
+    }
+  } else {
+    const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    for (int p = 0; p < 6; ++p)
+      for (int m = 0; m < 8; ++m) {
+        int v[3];
+        for (int a = 0; a < 3; ++a) {
+          const int t = q[perm[p][a]];
+          v[a] = (m >> a & 1) ? c.nx - 1 - t : t;
+        }
+        s += in[c.off + ((long long)v[0] * c.nx + v[1]) * c.nx + v[2]];
+        ++cnt;
+      }
+  }
+  out[c.off + n] = s / (double)cnt;
+}
+
+void symmetrize(const Launch& L, int kind, int axes, const double* in, double* out, hipStream_t s) {
+  k_symmetrize<<<L.nblocks, 256, 0, s>>>(L, kind, axes, in, out);
+}
+
 // ---- OC ----------------------------------------------------------------------
 
 // Midpoint of heap node t (1..63) of the depth-6 subtree rooted at bracket (l1, l2):

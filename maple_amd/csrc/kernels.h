@@ -165,6 +165,8 @@ void reduce_energy(const int* blk_start, int ncells, int nu, const double* parti
 void sensitivity(const Launch& L, int nu3, const double* Se3, const double* G6, int nu1, const double* Se1,
                  const double* G3, double* dJdrho, hipStream_t s);
 void box_filter(const Launch& L, int k, const double* in, double* out, hipStream_t s);
+// mean over reflections (kind 1, axes bit mask) or the 48 cube symmetries (kind 2)
+void symmetrize(const Launch& L, int kind, int axes, const double* in, double* out, hipStream_t s);
 struct OCState {
   double l1, l2, last_lam;
   double volfrac, move, xmin;

@@ -84,3 +84,50 @@ def run(r, x0, weights, iters, volfrac=0.25, nlevels=3, rtol=1e-6, move=0.2, xmi
         if verbose:
             print(f"it={it:04d} J={info['J']:.6e} mean_x={x.mean():.4f} dt={dt:.2f}s", flush=True)
     return x, hist, times
+
+
+def run_driver(r, x0, weights, iters, volfrac, symmetry="none", nlevels=3, rtol=1e-10, move=0.2, xmin=1e-3,
+               p=3.0, rho_min=1e-9, filter_k=3, tol_x=1e-3, tol_J=1e-4, patience=10, min_iters=10):
+    """The loop of optimize_inverse_homogenization (main_IHD_v3.py:603-758) without
+    penalties or meshes: symmetry projection of x (and, by the chain rule, of dJ/dx),
+    filter, OC, early stop.  Returns (x, rows) with rows
+    (iter, J, mean_x_used, mean_rho, dx_max, dx_mean, converged)."""
+    from .symmetry import parse_symmetry, symmetrize
+    kind, axes = parse_symmetry(symmetry)
+    S = (lambda v: symmetrize(v, r, kind, axes)) if kind != "none" else (lambda v: np.asarray(v, dtype=np.float64))
+    cs = CellSolver((r, r, r), nlevels=nlevels, rtol=rtol)
+    x = np.clip(S(np.asarray(x0, dtype=np.float64)), xmin, 1.0)
+    rows = []
+    J_prev = None
+    stall = 0
+    for it in range(iters):
+        x_old = x
+        xu = np.clip(S(np.clip(x, xmin, 1.0)), xmin, 1.0)
+        rho = simp(xu, p, rho_min)
+        need_ch = any(t in ("stiffness", "shear", "bulk") for t in weights)
+        need_k = "thermal" in weights
+        CH = S6 = kap = S3 = None
+        if need_k:
+            kap, S3, _ = cs.thermal(rho)
+        if need_ch:
+            CH, S6, _ = cs.elastic(rho)
+        J, G6, G3 = obj.evaluate(weights, CH, kap)
+        dJdrho = np.zeros_like(rho)
+        if need_ch:
+            dJdrho += np.einsum("ij,eij->e", G6, S6)
+        if need_k:
+            dJdrho += np.einsum("ij,eij->e", G3, S3)
+        dJdx = S(dJdrho * simp_grad(xu, p, rho_min))
+        if filter_k and filter_k > 1:
+            dJdx = box_filter(dJdx, cs.dims, filter_k)
+        xn, _ = oc_update(x_old, dJdx, volfrac, move, xmin)
+        x_next = np.clip(S(xn), xmin, 1.0)
+        dx = np.abs(x_next - x_old)
+        dJ_rel = float("inf") if J_prev is None else abs(J - J_prev) / max(abs(J_prev), 1e-12)
+        stall = stall + 1 if (it >= min_iters and dx.max() < tol_x and dJ_rel < tol_J) else 0
+        x = x_next
+        J_prev = J
+        rows.append((it + 1, J, xu.mean(), rho.mean(), dx.max(), dx.mean(), stall >= patience))
+        if stall >= patience:
+            break
+    return x, rows
