@@ -98,6 +98,12 @@ int maple_set_smoother(maple_ctx* ctx, int kind);
  * solves the cases one after another (model.py:111-124). */
 int maple_set_cg(maple_ctx* ctx, int flexible);
 
+/* When a homogenization asks for both physics, solve them at the same time:
+ * the thermal solve runs on a second stream of the context, driven by a second
+ * host thread, and the call returns when both are done (on = 1, the default), or
+ * one after the other on the caller's stream (on = 0). */
+int maple_set_concurrency(maple_ctx* ctx, int on);
+
 /* MAPLE_PREC_MIXED only: a solve that has not converged after `iters` CG
  * iterations builds the FP64 hierarchy of the same density and finishes on the
  * FP64 V-cycle (0 = never; default 100).  An FP32 V-cycle cannot resolve the

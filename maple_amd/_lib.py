@@ -45,6 +45,7 @@ SIGNATURES = {
     "maple_set_smoother": (_I, [_P, _I]),
     "maple_set_cg": (_I, [_P, _I]),
     "maple_set_mixed_fallback": (_I, [_P, _I]),
+    "maple_set_concurrency": (_I, [_P, _I]),
     "maple_set_coarse_refresh": (_I, [_P, _I]),
     "maple_set_variant": (_I, [_P, _I, _I]),
     "maple_homogenize": (_I, [_P, _P, _I, _D, _I, _P, _P, _IP, _P]),

@@ -121,6 +121,10 @@ class CellBatch:
         mode = 2 if block else (-1 if flexible is None else int(bool(flexible)))
         L.check(self.lib.maple_set_cg(self._h, mode))
 
+    def set_concurrency(self, on: bool = True):
+        """Solve elastic and thermal at the same time (second stream and host thread) or in turn."""
+        L.check(self.lib.maple_set_concurrency(self._h, int(bool(on))))
+
     def set_mixed_fallback(self, iters: int = 100):
         """Mixed form: finish a solve on the FP64 V-cycle once it has run `iters` CG
         iterations (0 = never)."""

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "element.h"
@@ -82,6 +83,8 @@ struct Phys {
   double* atol2 = nullptr;
   bool have_fields = false, have_energy = false;
   double* ustart = nullptr;   // supplied start of the next solve
+  double* partial = nullptr;  // reduction scratch of this physics' solve (the two solves may run concurrently)
+  hipEvent_t poll[2] = {nullptr, nullptr};
   BlockScalars* bs = nullptr; // block CG state, one per cell
   bool have_start = false;
   long long setups = 0;    // hierarchy builds so far (drives the coarsest-inverse refresh)
@@ -111,7 +114,9 @@ struct maple_ctx {
   OCState* ocst = nullptr;
   OCState* ocst_h = nullptr;
   double* oc_a = nullptr;
-  hipEvent_t poll[2] = {nullptr, nullptr};
+  hipStream_t side = nullptr;                  // the thermal solve's stream when both physics run
+  hipEvent_t fork = nullptr, join = nullptr;
+  bool concurrent = true;                      // solve elastic and thermal at the same time
   bool timing = false;
   Timer timers[3];   // 0: FP64 elastic level-0 operator, 1: FP64 thermal, 2: FP32 elastic
   std::vector<void*> allocs;
@@ -399,7 +404,7 @@ double* precondition(maple_ctx* c, Phys& P, const double* r, bool use64, hipStre
 }
 
 int poll_done(maple_ctx* c, Phys& P, int slot, bool* all) {
-  HIPCHK(hipEventSynchronize(c->poll[slot]));
+  HIPCHK(hipEventSynchronize(P.poll[slot]));
   const int n = c->ncells * P.nc;
   bool a = true;
   for (int i = 0; i < n; ++i) a = a && P.done_h[slot * n + i];
@@ -410,57 +415,57 @@ int poll_done(maple_ctx* c, Phys& P, int slot, bool* all) {
 int enqueue_poll(maple_ctx* c, Phys& P, int slot, hipStream_t s) {
   const int n = c->ncells * P.nc;
   HIPCHK(hipMemcpyAsync(P.done_h + slot * n, P.done_d, sizeof(int) * n, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipEventRecord(c->poll[slot], s));
+  HIPCHK(hipEventRecord(P.poll[slot], s));
   return 0;
 }
 
-int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStream_t s) {
+int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStream_t s, bool force_prev = false) {
   Level& L0 = c->lv[0];
   const bool flex = c->flexible < 0 ? c->prec != MAPLE_PREC_FP64 : c->flexible != 0;
   bool use64 = c->prec == MAPLE_PREC_FP64;
   const long long n0 = (long long)P.nc * P.nd * L0.Ntot;
   vec_zero<int>((long long)c->ncells * P.nc, P.done_d, s);   // no case is converged before the start
   fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
-  vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, c->partial, s);
-  reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_FF, s);
+  vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, P.partial, s);
+  reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_FF, s);
   // start: per (cell, case) the smallest residual among zero, the previous fields
   // (warm start) and a supplied start (learned warm start)
-  const bool have_prev = c->warm && P.have_fields;
+  const bool have_prev = (c->warm || force_prev) && P.have_fields;
   if (have_prev) {
     apply_fine64(c, P, 2, rho, P.u, P.f, P.r, nullptr, s);
-    vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, c->partial, s);
-    reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
+    vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, P.partial, s);
+    reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RR, s);
   }
   if (P.have_start) {
     apply_fine64(c, P, 2, rho, P.ustart, P.f, P.q, nullptr, s);
-    vec_dot<double, double>(L0.node, P.nd, P.nc, P.q, P.q, c->partial, s);
-    reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RRS, s);
+    vec_dot<double, double>(L0.node, P.nd, P.nc, P.q, P.q, P.partial, s);
+    reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RRS, s);
   }
   pcg_pick_start(L0.node, c->ncells, P.nd, P.nc, have_prev, P.have_start, P.sc, P.u, P.r, P.ustart, P.q, P.f, s);
   P.have_start = false;
-  vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, c->partial, s);
-  reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
+  vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, P.partial, s);
+  reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RR, s);
   double* z = precondition(c, P, P.r, use64, s);
-  vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, c->partial, s);
-  reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RZ_NEW, s);
+  vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, P.partial, s);
+  reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RZ_NEW, s);
   pcg_init_scalars(c->ncells, P.nc, rtol, P.atol2, P.sc, P.done_d, s);
   vec_copy<double, double>(n0, z, P.p, s);
   TRY(enqueue_poll(c, P, 0, s));
   bool all = false;
   TRY(poll_done(c, P, 0, &all));
   for (int it = 0; it < maxit && !all; ++it) {
-    apply_fine64(c, P, 1, rho, P.p, nullptr, P.q, c->partial, s);
-    reduce_partials(L0.fine.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_PQ, s);
+    apply_fine64(c, P, 1, rho, P.p, nullptr, P.q, P.partial, s);
+    reduce_partials(L0.fine.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_PQ, s);
     pcg_alpha(c->ncells, P.nc, P.sc, s);
-    pcg_update_ur(L0.node, P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, c->partial, s);
-    reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
+    pcg_update_ur(L0.node, P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, P.partial, s);
+    reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RR, s);
     pcg_check(c->ncells, P.nc, P.sc, P.done_d, s);
     const int slot = (it + 1) & 1;
     TRY(enqueue_poll(c, P, slot, s));
     if (it + 1 < maxit) {
       if (flex) {
-        vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, c->partial, s);
-        reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RZO, s);
+        vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, P.partial, s);
+        reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RZO, s);
       }
       // An FP32 V-cycle cannot resolve the soft modes of solid islands held by 1e-9
       // voids: a mixed solve still running after `fallback` iterations finishes on
@@ -470,8 +475,8 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
         use64 = true;
       }
       z = precondition(c, P, P.r, use64, s);
-      vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, c->partial, s);
-      reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RZ_NEW, s);
+      vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, P.partial, s);
+      reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RZ_NEW, s);
       pcg_beta(c->ncells, P.nc, flex, P.sc, s);
       pcg_update_p(L0.node, P.nd, P.nc, P.sc, z, P.p, s);
     }
@@ -493,37 +498,37 @@ int bpcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipSt
   bool use64 = c->prec == MAPLE_PREC_FP64;
   vec_zero<int>((long long)c->ncells * P.nc, P.done_d, s);
   fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
-  vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, c->partial, s);
-  reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_FF, s);
+  vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, P.partial, s);
+  reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_FF, s);
   const bool have_prev = c->warm && P.have_fields;
   if (have_prev) {
     apply_fine64(c, P, 2, rho, P.u, P.f, P.r, nullptr, s);
-    vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, c->partial, s);
-    reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
+    vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, P.partial, s);
+    reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RR, s);
   }
   if (P.have_start) {
     apply_fine64(c, P, 2, rho, P.ustart, P.f, P.q, nullptr, s);
-    vec_dot<double, double>(L0.node, P.nd, P.nc, P.q, P.q, c->partial, s);
-    reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RRS, s);
+    vec_dot<double, double>(L0.node, P.nd, P.nc, P.q, P.q, P.partial, s);
+    reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RRS, s);
   }
   pcg_pick_start(L0.node, c->ncells, P.nd, P.nc, have_prev, P.have_start, P.sc, P.u, P.r, P.ustart, P.q, P.f, s);
   P.have_start = false;
-  vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, c->partial, s);
-  reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
+  vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, P.partial, s);
+  reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RR, s);
   pcg_init_scalars(c->ncells, P.nc, rtol, P.atol2, P.sc, P.done_d, s);
   bcg_init(c->ncells, P.nc, P.bs, P.sc, P.done_d, s);
   double* z = precondition(c, P, P.r, use64, s);
-  bcg_gram(LN, P.nd, P.nc, 1, z, z, nullptr, c->partial, P.bs, s);   // translation content of Z
-  bcg_new_direction(LN, P.nd, P.nc, true, P.bs, z, P.p, c->partial, s);
+  bcg_gram(LN, P.nd, P.nc, 1, z, z, nullptr, P.partial, P.bs, s);   // translation content of Z
+  bcg_new_direction(LN, P.nd, P.nc, true, P.bs, z, P.p, P.partial, s);
   TRY(enqueue_poll(c, P, 0, s));
   bool all = false;
   TRY(poll_done(c, P, 0, &all));
   for (int it = 0; it < maxit && !all; ++it) {
-    apply_fine64(c, P, 1, rho, P.p, nullptr, P.q, c->partial, s);   // Q = K P (live cells only)
-    bcg_gram(LN, P.nd, P.nc, 2, P.p, P.q, P.r, c->partial, P.bs, s);
+    apply_fine64(c, P, 1, rho, P.p, nullptr, P.q, P.partial, s);   // Q = K P (live cells only)
+    bcg_gram(LN, P.nd, P.nc, 2, P.p, P.q, P.r, P.partial, P.bs, s);
     bcg_alpha(c->ncells, P.nc, P.bs, s);
-    bcg_update_xr(LN, P.nd, P.nc, P.bs, P.p, P.q, P.u, P.r, c->partial, s);
-    reduce_partials(L0.node.blk_start, c->ncells, P.nc, c->partial, P.sc, SC_RR, s);
+    bcg_update_xr(LN, P.nd, P.nc, P.bs, P.p, P.q, P.u, P.r, P.partial, s);
+    reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RR, s);
     bcg_check(c->ncells, P.nc, P.bs, P.sc, P.done_d, s);
     const int slot = (it + 1) & 1;
     TRY(enqueue_poll(c, P, slot, s));
@@ -533,9 +538,9 @@ int bpcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipSt
         use64 = true;
       }
       z = precondition(c, P, P.r, use64, s);
-      bcg_gram(LN, P.nd, P.nc, 1, z, P.q, nullptr, c->partial, P.bs, s);
+      bcg_gram(LN, P.nd, P.nc, 1, z, P.q, nullptr, P.partial, P.bs, s);
       bcg_beta(c->ncells, P.nc, P.bs, s);
-      bcg_new_direction(LN, P.nd, P.nc, false, P.bs, z, P.p, c->partial, s);
+      bcg_new_direction(LN, P.nd, P.nc, false, P.bs, z, P.p, P.partial, s);
     }
     TRY(poll_done(c, P, slot, &all));
   }
@@ -559,12 +564,32 @@ int solve_cases(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit
   if (rc != MAPLE_ENOCONV || nb >= maxit) return rc;
   std::vector<int> it0(P.hsc.size());
   for (size_t i = 0; i < P.hsc.size(); ++i) it0[i] = P.hsc[i].iters;
-  const int warm = c->warm;
-  c->warm = 1;
-  rc = pcg(c, P, rho, rtol, maxit - nb, s);
-  c->warm = warm;
+  rc = pcg(c, P, rho, rtol, maxit - nb, s, true);
   for (size_t i = 0; i < P.hsc.size(); ++i) P.hsc[i].iters += it0[i];
   return rc;
+}
+
+// One physics of a homogenization on stream s: hierarchy, cell solve, C_H (or
+// kappa_H) and the element energies behind the sensitivities.
+int solve_physics(maple_ctx* c, int p, const double* rho, double rtol, int maxit, double* out, int* iters_host,
+                  hipStream_t s) {
+  Phys& P = c->ph[p];
+  setup_hierarchy(c, P, rho, s);
+  HIPCHK(hipGetLastError());
+  int status = solve_cases(c, P, rho, rtol, maxit, s);
+  if (status == MAPLE_ENOCONV) {
+    g_err = std::string(p == 0 ? "elastic" : "thermal") + " CG reached maxit before rtol";
+  } else if (status) {
+    return status;
+  }
+  element_energy(c->lv[0].node, P.nd, P.nc, P.Ke, P.X0, rho, P.u, P.Se, P.partial, s);
+  if (out) reduce_energy(c->lv[0].node.blk_start, c->ncells, P.nc * (P.nc + 1) / 2, P.partial, P.nc, out, s);
+  HIPCHK(hipGetLastError());
+  P.have_energy = true;
+  if (iters_host)
+    for (int i = 0; i < c->ncells; ++i)
+      for (int j = 0; j < P.nc; ++j) iters_host[i * 9 + (p == 0 ? 0 : 6) + j] = P.hsc[i * P.nc + j].iters;
+  return status;
 }
 
 int check_ctx(const maple_ctx* c) {
@@ -685,9 +710,16 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
     return bail(fail(MAPLE_ENOMEM, "pinned host allocation failed"));
   c->host_allocs.push_back(hp);
   c->ocst_h = (OCState*)hp;
-  for (int k = 0; k < 2; ++k)
-    if (hipEventCreateWithFlags(&c->poll[k], hipEventDisableTiming) != hipSuccess)
-      return bail(fail(MAPLE_EDEVICE, "event creation failed"));
+  for (Phys& P : c->ph) {
+    if ((rc = dalloc(c, &P.partial, (size_t)maxb * 80))) return bail(rc);
+    for (int k = 0; k < 2; ++k)
+      if (hipEventCreateWithFlags(&P.poll[k], hipEventDisableTiming) != hipSuccess)
+        return bail(fail(MAPLE_EDEVICE, "event creation failed"));
+  }
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(MAPLE_EDEVICE, "stream or event creation failed"));
   *out = c;
   return 0;
 }
@@ -698,8 +730,12 @@ int maple_ctx_destroy(maple_ctx* c) {
   (void)hipDeviceSynchronize();
   for (void* p : c->allocs) (void)hipFree(p);
   for (void* p : c->host_allocs) (void)hipHostFree(p);
-  for (int k = 0; k < 2; ++k)
-    if (c->poll[k]) (void)hipEventDestroy(c->poll[k]);
+  for (Phys& P : c->ph)
+    for (int k = 0; k < 2; ++k)
+      if (P.poll[k]) (void)hipEventDestroy(P.poll[k]);
+  if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->join) (void)hipEventDestroy(c->join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   for (Timer& t : c->timers)
     for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
   delete c;
@@ -751,6 +787,12 @@ int maple_set_coarse_refresh(maple_ctx* c, int every) {
   return 0;
 }
 
+int maple_set_concurrency(maple_ctx* c, int on) {
+  if (!c) return fail(MAPLE_EINVAL, "null context");
+  c->concurrent = on != 0;
+  return 0;
+}
+
 int maple_set_mixed_fallback(maple_ctx* c, int iters) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
   if (iters < 0) return fail(MAPLE_EINVAL, "fallback iteration count must be >= 0");
@@ -779,29 +821,45 @@ int maple_homogenize(maple_ctx* c, const double* rho, int physics, double rtol, 
   if (!(physics & (MAPLE_ELASTIC | MAPLE_THERMAL))) return fail(MAPLE_EINVAL, "no physics selected");
   if (!(rtol > 0.0) || maxit < 1) return fail(MAPLE_EINVAL, "need rtol > 0 and maxit >= 1");
   hipStream_t s = (hipStream_t)stream;
-  int status = 0;
-  for (int p = 0; p < 2; ++p) {
-    if (!(physics & (p == 0 ? MAPLE_ELASTIC : MAPLE_THERMAL))) continue;
-    Phys& P = c->ph[p];
-    double* out = p == 0 ? CH : kappa;
-    setup_hierarchy(c, P, rho, s);
-    HIPCHK(hipGetLastError());
-    int rc = solve_cases(c, P, rho, rtol, maxit, s);
-    if (rc == MAPLE_ENOCONV) {
-      status = rc;
-      g_err = std::string(p == 0 ? "elastic" : "thermal") + " CG reached maxit before rtol";
-    } else if (rc) {
-      return rc;
+  const bool both = (physics & MAPLE_ELASTIC) && (physics & MAPLE_THERMAL);
+  if (!both || !c->concurrent) {
+    int status = 0;
+    for (int p = 0; p < 2; ++p) {
+      if (!(physics & (p == 0 ? MAPLE_ELASTIC : MAPLE_THERMAL))) continue;
+      const int rc = solve_physics(c, p, rho, rtol, maxit, p == 0 ? CH : kappa, iters_host, s);
+      if (rc == MAPLE_ENOCONV) status = rc;
+      else if (rc) return rc;
     }
-    element_energy(c->lv[0].node, P.nd, P.nc, P.Ke, P.X0, rho, P.u, P.Se, c->partial, s);
-    if (out) reduce_energy(c->lv[0].node.blk_start, c->ncells, P.nc * (P.nc + 1) / 2, c->partial, P.nc, out, s);
-    HIPCHK(hipGetLastError());
-    P.have_energy = true;
-    if (iters_host)
-      for (int i = 0; i < c->ncells; ++i)
-        for (int j = 0; j < P.nc; ++j) iters_host[i * 9 + (p == 0 ? 0 : 6) + j] = P.hsc[i * P.nc + j].iters;
+    return status;
   }
-  return status;
+  // The two physics share nothing but rho: the thermal solve runs on a second
+  // stream, driven by a second host thread (each CG loop polls its own flags),
+  // so its small kernels fill the CUs the elastic kernels leave idle.
+  HIPCHK(hipEventRecord(c->fork, s));
+  HIPCHK(hipStreamWaitEvent(c->side, c->fork, 0));
+  int rc_th = 0;
+  std::string err_th;
+  std::thread th([&] {
+    if (hipSetDevice(c->device) != hipSuccess) {
+      rc_th = MAPLE_EDEVICE;
+      err_th = "hipSetDevice failed";
+      return;
+    }
+    rc_th = solve_physics(c, 1, rho, rtol, maxit, kappa, iters_host, c->side);
+    if (rc_th) err_th = g_err;
+  });
+  const int rc_el = solve_physics(c, 0, rho, rtol, maxit, CH, iters_host, s);
+  const std::string err_el = g_err;
+  th.join();
+  HIPCHK(hipEventRecord(c->join, c->side));
+  HIPCHK(hipStreamWaitEvent(s, c->join, 0));
+  if (rc_el && rc_el != MAPLE_ENOCONV) return fail(rc_el, err_el);
+  if (rc_th && rc_th != MAPLE_ENOCONV) return fail(rc_th, err_th);
+  if (rc_el || rc_th) {
+    g_err = rc_el ? err_el : err_th;
+    return MAPLE_ENOCONV;
+  }
+  return 0;
 }
 
 int maple_sensitivity(maple_ctx* c, const double* G6, const double* G3, double* out, void* stream) {

@@ -253,3 +253,21 @@ def test_batched_oc_steps_equal_single_cells():
             xi, _ = d1.step(xi)
         assert np.abs(x[o:o + r ** 3].cpu().numpy() - xi.cpu().numpy()).max() <= 1e-9
         o += r ** 3
+
+
+def test_concurrent_physics_equal_sequential():
+    """Elastic and thermal solved at the same time (second stream and host thread) give
+    bitwise the results of solving them one after the other."""
+    r = 32
+    rho = _t(_rho(r, 9))
+    a = _cells([r, 16], 3)
+    b = _cells([r, 16], 3)
+    b.set_concurrency(False)
+    rr = torch.cat([rho, _t(_rho(16, 10))])
+    CHa, ka, ia = a.homogenize(rr, rtol=1e-8)
+    CHb, kb, ib = b.homogenize(rr, rtol=1e-8)
+    np.testing.assert_array_equal(ia, ib)
+    assert torch.equal(CHa, CHb) and torch.equal(ka, kb)
+    da = a.sensitivity(torch.ones(2, 6, 6, dtype=torch.float64, device=DEV), torch.ones(2, 3, 3, dtype=torch.float64, device=DEV))
+    db = b.sensitivity(torch.ones(2, 6, 6, dtype=torch.float64, device=DEV), torch.ones(2, 3, 3, dtype=torch.float64, device=DEV))
+    assert torch.equal(da, db)
