@@ -20,10 +20,13 @@ namespace {
 
 constexpr int KG = 36;   // one NC x NC Gram matrix (NC <= 6), row-major
 
-__device__ __forceinline__ bool bnode(const Launch& L, int& ci, long long& gid) {
-  ci = find_cell(L.blk_start, L.ncells, lbid());
-  const LevelCell c = L.cells[ci];
-  const long long n = (long long)(lbid() - L.blk_start[ci]) * 256 + threadIdx.x;
+// ci: the id of the block's cell (per-cell arrays); gid: the thread's node
+__device__ __forceinline__ bool bnode(const Launch& L, int& ci, long long& gid, int* kc = nullptr) {
+  const int k = find_cell(L.blk_start, L.ncells, lbid());
+  const LevelCell c = L.cells[k];
+  ci = c.id;
+  if (kc) *kc = k;
+  const long long n = (long long)(lbid() - L.blk_start[k]) * 256 + threadIdx.x;
   if (n >= (long long)c.nx * c.ny * c.nz) return false;
   gid = c.off + n;
   return true;
@@ -69,11 +72,12 @@ __global__ __launch_bounds__(256) void k_gram(Launch L, const double* __restrict
 
 // Reduce the k-th partial of every cell over its blocks: Gram entries (k < ngram*per)
 // go to gram[(k/per)*36 + k%per], the component sums after them to gram[36 + ...].
-__global__ __launch_bounds__(64) void k_gram_reduce(const int* __restrict__ blk_start, const double* __restrict__ partial,
+__global__ __launch_bounds__(64) void k_gram_reduce(const LevelCell* __restrict__ cells,
+                                                    const int* __restrict__ blk_start, const double* __restrict__ partial,
                                                     int per, int ngram, double* __restrict__ out, int stride,
                                                     int offset) {
-  const int ci = blockIdx.x, k = blockIdx.y;
-  const int b0 = blk_start[ci], b1 = blk_start[ci + 1];
+  const int kcell = blockIdx.x, k = blockIdx.y, ci = cells[kcell].id;
+  const int b0 = blk_start[kcell], b1 = blk_start[kcell + 1];
   double v = 0.0;
   for (int b = b0 + (int)threadIdx.x; b < b1; b += 64) v += partial[(long long)b * (2 * KG) + k];
   v = wave_sum(v);
@@ -231,15 +235,15 @@ template <int NC, int ND>
 __global__ __launch_bounds__(256) void k_bcg_v(Launch L, const BlockScalars* __restrict__ bs, int first,
                                                const double* __restrict__ Z, double* __restrict__ P,
                                                double* __restrict__ partial) {
-  int ci;
+  int ci, kc;
   long long gid = 0;
-  const bool ok = bnode(L, ci, gid);
+  const bool ok = bnode(L, ci, gid, &kc);
   double acc[NC * NC];
 #pragma unroll
   for (int k = 0; k < NC * NC; ++k) acc[k] = 0.0;
   if (ok && !bs[ci].done) {
     const double* be = bs[ci].beta;
-    const LevelCell cc = L.cells[ci];
+    const LevelCell cc = L.cells[kc];
     const double inv_n = 1.0 / ((double)cc.nx * cc.ny * cc.nz);
 #pragma unroll
     for (int c = 0; c < ND; ++c) {
@@ -370,7 +374,7 @@ void gram_t(const Launch& L, int ng, const double* A, const double* B0, const do
     k_gram<NC, ND, 2, false><<<L.nblocks, 256, 0, s>>>(L, A, B0, B1, partial);
     nv = 2 * NC * NC;
   }
-  k_gram_reduce<<<dim3(L.ncells, nv), 64, 0, s>>>(L.blk_start, partial, NC * NC, ng, out,
+  k_gram_reduce<<<dim3(L.ncells, nv), 64, 0, s>>>(L.cells, L.blk_start, partial, NC * NC, ng, out,
                                                   (int)(sizeof(BlockScalars) / sizeof(double)), offset);
 }
 
@@ -422,12 +426,12 @@ void bcg_new_direction(const Launch& L, int nd, int nc, bool first, BlockScalars
   const int nb = (L.ncells + 63) / 64;
   if (nc == 6 && nd == 3) {
     k_bcg_v<6, 3><<<L.nblocks, 256, 0, s>>>(L, bs, first ? 1 : 0, Z, P, partial);
-    k_gram_reduce<<<dim3(L.ncells, 36), 64, 0, s>>>(L.blk_start, partial, 36, 1, out, stride, off);
+    k_gram_reduce<<<dim3(L.ncells, 36), 64, 0, s>>>(L.cells, L.blk_start, partial, 36, 1, out, stride, off);
     k_bcg_orth<6><<<nb, 64, 0, s>>>(L.ncells, bs);
     k_bcg_p<6, 3><<<L.nblocks, 256, 0, s>>>(L, bs, P);
   } else {
     k_bcg_v<3, 1><<<L.nblocks, 256, 0, s>>>(L, bs, first ? 1 : 0, Z, P, partial);
-    k_gram_reduce<<<dim3(L.ncells, 9), 64, 0, s>>>(L.blk_start, partial, 9, 1, out, stride, off);
+    k_gram_reduce<<<dim3(L.ncells, 9), 64, 0, s>>>(L.cells, L.blk_start, partial, 9, 1, out, stride, off);
     k_bcg_orth<3><<<nb, 64, 0, s>>>(L.ncells, bs);
     k_bcg_p<3, 1><<<L.nblocks, 256, 0, s>>>(L, bs, P);
   }

@@ -50,6 +50,7 @@ struct Level {
   int* dfblk = nullptr;   // level 0: operator blocks (4 waves)
   int* doblk = nullptr;   // level 0: 1024-element blocks (OC)
   Launch node, fine, oc;
+  std::vector<int> cnt_node, cnt_fine;   // blocks per cell of the node and operator launches
 };
 
 // Multigrid hierarchy and V-cycle work vectors in precision T (FP64 for the
@@ -86,6 +87,15 @@ struct Phys {
   double* partial = nullptr;  // reduction scratch of this physics' solve (the two solves may run concurrently)
   hipEvent_t poll[2] = {nullptr, nullptr};
   BlockScalars* bs = nullptr; // block CG state, one per cell
+  // launch tables listing only the cells still iterating (per level, and the
+  // level-0 operator's), their pinned host staging, and how many cells they list
+  std::vector<LevelCell*> cc;
+  std::vector<int*> cb;
+  LevelCell* ccf = nullptr;
+  int* cbf = nullptr;
+  char* stage = nullptr;
+  size_t stage_bytes = 0;
+  int stage_slots = 0, stage_next = 0, nlive = 0;
   bool have_start = false;
   long long setups = 0;    // hierarchy builds so far (drives the coarsest-inverse refresh)
 };
@@ -142,12 +152,15 @@ int set_device(const maple_ctx* c) {
   return 0;
 }
 
-int make_launch(maple_ctx* c, Level& L, int per_block, bool fine_waves_mode, int** dptr, Launch* out) {
+int make_launch(maple_ctx* c, Level& L, int per_block, bool fine_waves_mode, int** dptr, Launch* out,
+                std::vector<int>* counts = nullptr) {
   std::vector<int> blk(c->ncells + 1, 0);
+  if (counts) counts->assign(c->ncells, 0);
   for (int i = 0; i < c->ncells; ++i) {
     const LevelCell& lc = L.h[i];
     long long n = fine_waves_mode ? (long long)fine_waves(lc) : (long long)lc.nx * lc.ny * lc.nz;
     blk[i + 1] = blk[i] + (int)((n + per_block - 1) / per_block);
+    if (counts) (*counts)[i] = blk[i + 1] - blk[i];
   }
   TRY(dalloc(c, dptr, blk.size()));
   HIPCHK(hipMemcpy(*dptr, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -267,6 +280,22 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
   P.lf = c->lv[0].fine;
   P.lf.done = P.done_d;
   P.lf.ndc = nc;
+  P.cc.assign(L, nullptr);
+  P.cb.assign(L, nullptr);
+  for (int l = 0; l < L; ++l) {
+    TRY(dalloc(c, &P.cc[l], (size_t)c->ncells));
+    TRY(dalloc(c, &P.cb[l], (size_t)c->ncells + 1));
+  }
+  TRY(dalloc(c, &P.ccf, (size_t)c->ncells));
+  TRY(dalloc(c, &P.cbf, (size_t)c->ncells + 1));
+  auto r8 = [](size_t b) { return (b + 7) / 8 * 8; };
+  P.stage_bytes = (size_t)(L + 1) * (r8(sizeof(LevelCell) * c->ncells) + r8(sizeof(int) * (c->ncells + 1)));
+  P.stage_slots = c->ncells + 2;   // a solve compacts at most ncells - 1 times: no slot is reused in flight
+  void* sp = nullptr;
+  HIPCHK(hipHostMalloc(&sp, P.stage_bytes * P.stage_slots, hipHostMallocDefault));
+  c->host_allocs.push_back(sp);
+  P.stage = (char*)sp;
+  P.nlive = c->ncells;
   P.hsc.resize((size_t)c->ncells * nc);
   void* hp = nullptr;
   HIPCHK(hipHostMalloc(&hp, sizeof(int) * 2 * c->ncells * nc, hipHostMallocDefault));
@@ -396,11 +425,63 @@ T* vcycle_t(maple_ctx* c, Phys& P, VC<T>& V, int l, const T* b, hipStream_t s) {
 // FP32 one between conversions.
 double* precondition(maple_ctx* c, Phys& P, const double* r, bool use64, hipStream_t s) {
   if (use64) return vcycle_t<double>(c, P, P.v64, 0, r, s);
-  const long long n0 = (long long)P.nc * P.nd * c->lv[0].Ntot;
-  vec_copy<float, double>(n0, r, P.v32.B[0], s);
+  vec_copy_cases<float, double>(P.ln[0], P.nd, P.nc, r, P.v32.B[0], s);
   float* z = vcycle_t<float>(c, P, P.v32, 0, P.v32.B[0], s);
-  vec_copy<double, float>(n0, z, P.z64, s);
+  vec_copy_cases<double, float>(P.ln[0], P.nd, P.nc, z, P.z64, s);
   return P.z64;
+}
+
+// Launch tables over every cell (start of a solve).
+void full_launches(maple_ctx* c, Phys& P) {
+  for (size_t l = 0; l < P.ln.size(); ++l) {
+    P.ln[l] = c->lv[l].node;
+    P.ln[l].done = P.done_d;
+    P.ln[l].ndc = P.nc;
+  }
+  P.lf = c->lv[0].fine;
+  P.lf.done = P.done_d;
+  P.lf.ndc = P.nc;
+  P.nlive = c->ncells;
+}
+
+// Once cells have converged (host flags of the last poll), the iteration's launches
+// list only the cells still iterating: a converged cell's blocks would only exit at
+// once, but a 16-cell batch with one live cell would still dispatch sixteen cells'
+// worth of them on every launch.  The tables are uploaded on the solve's stream, so
+// the kernels already queued keep reading the previous ones.
+int compact_launches(maple_ctx* c, Phys& P, const int* done_h, hipStream_t s) {
+  std::vector<int> live;
+  for (int i = 0; i < c->ncells; ++i) {
+    bool d = true;
+    for (int j = 0; j < P.nc; ++j) d = d && done_h[i * P.nc + j];
+    if (!d) live.push_back(i);
+  }
+  const int n = (int)live.size();
+  if (n == 0 || n >= P.nlive) return 0;
+  char* st = P.stage + (size_t)(P.stage_next++ % P.stage_slots) * P.stage_bytes;
+  auto r8 = [](size_t b) { return (b + 7) / 8 * 8; };
+  auto build = [&](const Level& Lv, const std::vector<int>& cnt, LevelCell* dcells, int* dblk, Launch& out) -> int {
+    LevelCell* hc = (LevelCell*)st;
+    st += r8(sizeof(LevelCell) * c->ncells);
+    int* hb = (int*)st;
+    st += r8(sizeof(int) * (c->ncells + 1));
+    hb[0] = 0;
+    for (int k = 0; k < n; ++k) {
+      hc[k] = Lv.h[live[k]];
+      hb[k + 1] = hb[k] + cnt[live[k]];
+    }
+    HIPCHK(hipMemcpyAsync(dcells, hc, sizeof(LevelCell) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dblk, hb, sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    out.cells = dcells;
+    out.blk_start = dblk;
+    out.ncells = n;
+    out.nblocks = hb[n];
+    return 0;
+  };
+  for (size_t l = 0; l < P.ln.size(); ++l) TRY(build(c->lv[l], c->lv[l].cnt_node, P.cc[l], P.cb[l], P.ln[l]));
+  TRY(build(c->lv[0], c->lv[0].cnt_fine, P.ccf, P.cbf, P.lf));
+  P.nlive = n;
+  return 0;
 }
 
 int poll_done(maple_ctx* c, Phys& P, int slot, bool* all) {
@@ -425,29 +506,30 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   bool use64 = c->prec == MAPLE_PREC_FP64;
   const long long n0 = (long long)P.nc * P.nd * L0.Ntot;
   vec_zero<int>((long long)c->ncells * P.nc, P.done_d, s);   // no case is converged before the start
+  full_launches(c, P);
   fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, P.partial, s);
-  reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_FF, s);
+  reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_FF, s);
   // start: per (cell, case) the smallest residual among zero, the previous fields
   // (warm start) and a supplied start (learned warm start)
   const bool have_prev = (c->warm || force_prev) && P.have_fields;
   if (have_prev) {
     apply_fine64(c, P, 2, rho, P.u, P.f, P.r, nullptr, s);
     vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, P.partial, s);
-    reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RR, s);
+    reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RR, s);
   }
   if (P.have_start) {
     apply_fine64(c, P, 2, rho, P.ustart, P.f, P.q, nullptr, s);
     vec_dot<double, double>(L0.node, P.nd, P.nc, P.q, P.q, P.partial, s);
-    reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RRS, s);
+    reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RRS, s);
   }
   pcg_pick_start(L0.node, c->ncells, P.nd, P.nc, have_prev, P.have_start, P.sc, P.u, P.r, P.ustart, P.q, P.f, s);
   P.have_start = false;
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, P.partial, s);
-  reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RR, s);
+  reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RR, s);
   double* z = precondition(c, P, P.r, use64, s);
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, P.partial, s);
-  reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RZ_NEW, s);
+  reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RZ_NEW, s);
   pcg_init_scalars(c->ncells, P.nc, rtol, P.atol2, P.sc, P.done_d, s);
   vec_copy<double, double>(n0, z, P.p, s);
   TRY(enqueue_poll(c, P, 0, s));
@@ -455,17 +537,17 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   TRY(poll_done(c, P, 0, &all));
   for (int it = 0; it < maxit && !all; ++it) {
     apply_fine64(c, P, 1, rho, P.p, nullptr, P.q, P.partial, s);
-    reduce_partials(L0.fine.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_PQ, s);
+    reduce_partials(P.lf, P.nc, P.partial, P.sc, SC_PQ, s);
     pcg_alpha(c->ncells, P.nc, P.sc, s);
-    pcg_update_ur(L0.node, P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, P.partial, s);
-    reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RR, s);
+    pcg_update_ur(P.ln[0], P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, P.partial, s);
+    reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RR, s);
     pcg_check(c->ncells, P.nc, P.sc, P.done_d, s);
     const int slot = (it + 1) & 1;
     TRY(enqueue_poll(c, P, slot, s));
     if (it + 1 < maxit) {
       if (flex) {
-        vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, P.partial, s);
-        reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RZO, s);
+        vec_dot<double, double>(P.ln[0], P.nd, P.nc, P.r, z, P.partial, s);
+        reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RZO, s);
       }
       // An FP32 V-cycle cannot resolve the soft modes of solid islands held by 1e-9
       // voids: a mixed solve still running after `fallback` iterations finishes on
@@ -475,13 +557,14 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
         use64 = true;
       }
       z = precondition(c, P, P.r, use64, s);
-      vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, P.partial, s);
-      reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RZ_NEW, s);
+      vec_dot<double, double>(P.ln[0], P.nd, P.nc, P.r, z, P.partial, s);
+      reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RZ_NEW, s);
       pcg_beta(c->ncells, P.nc, flex, P.sc, s);
-      pcg_update_p(L0.node, P.nd, P.nc, P.sc, z, P.p, s);
+      pcg_update_p(P.ln[0], P.nd, P.nc, P.sc, z, P.p, s);
     }
     // the device is already working on the next iteration while we look at this one
     TRY(poll_done(c, P, slot, &all));
+    if (!all) TRY(compact_launches(c, P, P.done_h + slot * c->ncells * P.nc, s));
   }
   P.have_fields = true;
   HIPCHK(hipMemcpyAsync(P.hsc.data(), P.sc, sizeof(CaseScalars) * P.hsc.size(), hipMemcpyDeviceToHost, s));
@@ -497,24 +580,25 @@ int bpcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipSt
   const Launch& LN = P.ln[0];
   bool use64 = c->prec == MAPLE_PREC_FP64;
   vec_zero<int>((long long)c->ncells * P.nc, P.done_d, s);
+  full_launches(c, P);
   fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, P.partial, s);
-  reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_FF, s);
+  reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_FF, s);
   const bool have_prev = c->warm && P.have_fields;
   if (have_prev) {
     apply_fine64(c, P, 2, rho, P.u, P.f, P.r, nullptr, s);
     vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, P.partial, s);
-    reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RR, s);
+    reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RR, s);
   }
   if (P.have_start) {
     apply_fine64(c, P, 2, rho, P.ustart, P.f, P.q, nullptr, s);
     vec_dot<double, double>(L0.node, P.nd, P.nc, P.q, P.q, P.partial, s);
-    reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RRS, s);
+    reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RRS, s);
   }
   pcg_pick_start(L0.node, c->ncells, P.nd, P.nc, have_prev, P.have_start, P.sc, P.u, P.r, P.ustart, P.q, P.f, s);
   P.have_start = false;
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, P.partial, s);
-  reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RR, s);
+  reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RR, s);
   pcg_init_scalars(c->ncells, P.nc, rtol, P.atol2, P.sc, P.done_d, s);
   bcg_init(c->ncells, P.nc, P.bs, P.sc, P.done_d, s);
   double* z = precondition(c, P, P.r, use64, s);
@@ -528,7 +612,7 @@ int bpcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipSt
     bcg_gram(LN, P.nd, P.nc, 2, P.p, P.q, P.r, P.partial, P.bs, s);
     bcg_alpha(c->ncells, P.nc, P.bs, s);
     bcg_update_xr(LN, P.nd, P.nc, P.bs, P.p, P.q, P.u, P.r, P.partial, s);
-    reduce_partials(L0.node.blk_start, c->ncells, P.nc, P.partial, P.sc, SC_RR, s);
+    reduce_partials(LN, P.nc, P.partial, P.sc, SC_RR, s);
     bcg_check(c->ncells, P.nc, P.bs, P.sc, P.done_d, s);
     const int slot = (it + 1) & 1;
     TRY(enqueue_poll(c, P, slot, s));
@@ -543,6 +627,7 @@ int bpcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipSt
       bcg_new_direction(LN, P.nd, P.nc, false, P.bs, z, P.p, P.partial, s);
     }
     TRY(poll_done(c, P, slot, &all));
+    if (!all) TRY(compact_launches(c, P, P.done_h + slot * c->ncells * P.nc, s));
   }
   P.have_fields = true;
   HIPCHK(hipMemcpyAsync(P.hsc.data(), P.sc, sizeof(CaseScalars) * P.hsc.size(), hipMemcpyDeviceToHost, s));
@@ -668,7 +753,7 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
       lc.nx = c->dims[3 * i] >> l;
       lc.ny = c->dims[3 * i + 1] >> l;
       lc.nz = c->dims[3 * i + 2] >> l;
-      lc.pad_ = 0;
+      lc.id = i;
       lc.off = off;
       off += (long long)lc.nx * lc.ny * lc.nz;
       L.h[i] = lc;
@@ -677,9 +762,9 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
     if ((rc = dalloc(c, &L.d, ncells))) return bail(rc);
     if (hipMemcpy(L.d, L.h.data(), sizeof(LevelCell) * ncells, hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(MAPLE_EDEVICE, "upload of the cell table failed"));
-    if ((rc = make_launch(c, L, 256, false, &L.dblk, &L.node))) return bail(rc);
+    if ((rc = make_launch(c, L, 256, false, &L.dblk, &L.node, &L.cnt_node))) return bail(rc);
     if (l == 0) {
-      if ((rc = make_launch(c, L, 4, true, &L.dfblk, &L.fine))) return bail(rc);
+      if ((rc = make_launch(c, L, 4, true, &L.dfblk, &L.fine, &L.cnt_fine))) return bail(rc);
       if ((rc = make_launch(c, L, 1024, false, &L.doblk, &L.oc))) return bail(rc);
     }
   }

@@ -13,11 +13,16 @@
 
 namespace {
 
+// ci: the cell's id (per-cell arrays); kc: its position in this launch's list (the
+// paired fine/coarse tables of a transfer list the same cells in the same order)
 __device__ __forceinline__ bool node_of(const LevelCell* cells, const int* blk_start, int ncells, int bid,
-                                        LevelCell& c, int& ci, int& x, int& y, int& z, long long& gid) {
-  ci = find_cell(blk_start, ncells, bid);
-  c = cells[ci];
-  const long long n = (long long)(bid - blk_start[ci]) * 256 + threadIdx.x;
+                                        LevelCell& c, int& ci, int& x, int& y, int& z, long long& gid,
+                                        int* kc = nullptr) {
+  const int k = find_cell(blk_start, ncells, bid);
+  c = cells[k];
+  ci = c.id;
+  if (kc) *kc = k;
+  const long long n = (long long)(bid - blk_start[k]) * 256 + threadIdx.x;
   const long long nn = (long long)c.nx * c.ny * c.nz;
   if (n >= nn) return false;
   z = (int)(n % c.nz);
@@ -143,8 +148,9 @@ __global__ __launch_bounds__(256) void k_stencil_level1(const LevelCell* __restr
                                                         T* __restrict__ S) {
   constexpr int NE = 8 * ND;
   LevelCell c; int ci, x, y, z; long long gid;
-  if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
-  const LevelCell cf = fcells[ci];
+  int kc;
+  if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid, &kc)) return;
+  const LevelCell cf = fcells[kc];
   const int d = blockIdx.y;
   const int dd[3] = {d / 9 - 1, (d / 3) % 3 - 1, d % 3 - 1};
   double acc[ND][ND];
@@ -195,8 +201,9 @@ __global__ __launch_bounds__(256) void k_stencil_galerkin(const LevelCell* __res
                                                           long long Ntot_f, const T* __restrict__ Sf,
                                                           T* __restrict__ Sc) {
   LevelCell c; int ci, x, y, z; long long gid;
-  if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
-  const LevelCell cf = fcells[ci];
+  int kc;
+  if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid, &kc)) return;
+  const LevelCell cf = fcells[kc];
   const int D = blockIdx.y;
   const int Dx = D / 9 - 1, Dy = (D / 3) % 3 - 1, Dz = D % 3 - 1;
   double acc[ND][ND];
@@ -252,9 +259,10 @@ __global__ __launch_bounds__(256) void k_restrict(const LevelCell* __restrict__ 
                                                   const T* __restrict__ rf, T* __restrict__ bc,
                                                   const int* __restrict__ done, int ndc, int nd) {
   LevelCell c; int ci, x, y, z; long long gid;
-  if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
+  int kc;
+  if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid, &kc)) return;
   if (case_done(done, ndc, ci, blockIdx.y / nd)) return;
-  const LevelCell cf = fcells[ci];
+  const LevelCell cf = fcells[kc];
   const T* __restrict__ r = rf + (long long)blockIdx.y * Ntot_f + cf.off;
   const int fx[3] = {wrap(2 * x - 1, cf.nx), 2 * x, 2 * x + 1};
   const int fy[3] = {wrap(2 * y - 1, cf.ny), 2 * y, 2 * y + 1};
@@ -290,9 +298,10 @@ __global__ __launch_bounds__(256) void k_prolong_add(const LevelCell* __restrict
                                                      const T* __restrict__ ec, T* __restrict__ uf,
                                                      const int* __restrict__ done, int ndc, int nd) {
   LevelCell c; int ci, x, y, z; long long gid;
-  if (!node_of(fcells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
+  int kc;
+  if (!node_of(fcells, blk_start, ncells, lbid(), c, ci, x, y, z, gid, &kc)) return;
   if (case_done(done, ndc, ci, blockIdx.y / nd)) return;
-  const LevelCell cc = ccells[ci];
+  const LevelCell cc = ccells[kc];
   const T* __restrict__ e = ec + (long long)blockIdx.y * Ntot_c + cc.off;
   const int bx = x >> 1, by = y >> 1, bz = z >> 1;
   const int rx = x & 1, ry = y & 1, rz = z & 1;

@@ -17,7 +17,7 @@
 // One cell at one multigrid level.
 struct LevelCell {
   int nx, ny, nz;
-  int pad_;
+  int id;          // index of the cell in the batch (per-cell arrays); a launch may list a subset of cells
   long long off;   // first node (= first element) of this cell in the level arrays
 };
 
@@ -46,8 +46,9 @@ __host__ __device__ inline int xcd_swizzle(int bid, int nb) {
 }
 __device__ __forceinline__ int lbid() { return xcd_swizzle((int)blockIdx.x, (int)gridDim.x); }
 
-// Block -> cell lookup: blk_start has ncells+1 entries, cell i owns blocks
-// [blk_start[i], blk_start[i+1]).
+// Block -> cell lookup: blk_start has ncells+1 entries, the i-th listed cell owns
+// blocks [blk_start[i], blk_start[i+1]).  A launch may list only the cells still
+// iterating; per-cell arrays are indexed by LevelCell::id, never by the position.
 __device__ __forceinline__ int find_cell(const int* __restrict__ blk_start, int ncells, int b) {
   int lo = 0, hi = ncells - 1;
   while (lo < hi) {

@@ -262,11 +262,12 @@ void dense_inverse(const DenseCell* dcells, int ncells, int max_nt, double* A, d
 
 // x_j = Ainv b_j for every case; one block of 256 threads per matrix row.
 template <int ND, typename T>
-__global__ __launch_bounds__(256) void k_dense_apply(const DenseCell* __restrict__ dcells, long long Ntot,
+__global__ __launch_bounds__(256) void k_dense_apply(const LevelCell* __restrict__ cells,
+                                                     const DenseCell* __restrict__ dcells, long long Ntot,
                                                      int ncase, const double* __restrict__ Ainv,
                                                      const T* __restrict__ b, T* __restrict__ x,
                                                      const int* __restrict__ done) {
-  const int ci = blockIdx.y;
+  const int ci = cells[blockIdx.y].id;
   const DenseCell dc = dcells[ci];
   const int row = blockIdx.x;
   if (row >= dc.n) return;
@@ -292,8 +293,8 @@ template <typename T>
 void dense_apply(const Launch& Lc, const DenseCell* dcells, int max_n, int nd, int ncase, const double* Ainv,
                  const T* b, T* x, hipStream_t s) {
   dim3 grid(max_n, Lc.ncells);
-  if (nd == 3) k_dense_apply<3, T><<<grid, 256, 0, s>>>(dcells, Lc.Ntot, ncase, Ainv, b, x, Lc.done);
-  else k_dense_apply<1, T><<<grid, 256, 0, s>>>(dcells, Lc.Ntot, ncase, Ainv, b, x, Lc.done);
+  if (nd == 3) k_dense_apply<3, T><<<grid, 256, 0, s>>>(Lc.cells, dcells, Lc.Ntot, ncase, Ainv, b, x, Lc.done);
+  else k_dense_apply<1, T><<<grid, 256, 0, s>>>(Lc.cells, dcells, Lc.Ntot, ncase, Ainv, b, x, Lc.done);
 }
 template void dense_apply<double>(const Launch&, const DenseCell*, int, int, int, const double*, const double*, double*, hipStream_t);
 template void dense_apply<float>(const Launch&, const DenseCell*, int, int, int, const double*, const float*, float*, hipStream_t);

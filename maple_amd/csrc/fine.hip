@@ -79,13 +79,13 @@ __global__ __launch_bounds__(256) void k_fine_apply(
     const T* __restrict__ u, const T* __restrict__ f, const T* __restrict__ dinv, T omega,
     T* __restrict__ out, double* __restrict__ partial, const int* __restrict__ done, int ndc) {
   constexpr int NE = 8 * ND;
-  const int ci = find_cell(blk_start, ncells, lbid());
-  if (case_done(done, ndc, ci, blockIdx.y)) return;   // converged: nothing of this case is read again
-  const LevelCell c = cells[ci];
-  const int wave = (lbid() - blk_start[ci]) * 4 + (threadIdx.x >> 6);
+  const int k = find_cell(blk_start, ncells, lbid());
+  const LevelCell c = cells[k];
+  if (case_done(done, ndc, c.id, blockIdx.y)) return;   // converged: nothing of this case is read again
+  const int wave = (lbid() - blk_start[k]) * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const LaneMap m = lane_map(c, wave, lane);
-  Ke += (long long)ci * NE * NE;   // per-cell element matrix (h = 1/r scaling)
+  Ke += (long long)c.id * NE * NE;   // per-cell element matrix (h = 1/r scaling)
 
   // Every lane (also an idle one) holds in-range coordinates, so loads need no
   // guard; only the store is masked.
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void k_fine_rhs(const LevelCell* __restrict__ 
                                                   const double* __restrict__ rho, double* __restrict__ F) {
   LevelCell c; int x, y, z; long long gid;
   if (!node_of(cells, blk_start, ncells, c, x, y, z, gid)) return;
-  Fe += (long long)find_cell(blk_start, ncells, lbid()) * 8 * ND * ncase;
+  Fe += (long long)c.id * 8 * ND * ncase;
   double acc[MAPLE_MAX_CASES][ND];
 #pragma unroll
   for (int j = 0; j < MAPLE_MAX_CASES; ++j)
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void k_fine_dinv(const LevelCell* __restrict__
                                                    const double* __restrict__ rho, T* __restrict__ dinv) {
   LevelCell c; int x, y, z; long long gid;
   if (!node_of(cells, blk_start, ncells, c, x, y, z, gid)) return;
-  Ke += (long long)find_cell(blk_start, ncells, lbid()) * 64 * ND * ND;
+  Ke += (long long)c.id * 64 * ND * ND;
   double acc[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) acc[d] = 0.0;

@@ -125,8 +125,8 @@ void bcg_new_direction(const Launch& L, int nd, int nc, bool first, BlockScalars
 void bcg_init(int ncells, int nc, BlockScalars* bs, CaseScalars* sc, int* done_flags, hipStream_t s);
 
 // sum partial[blk*6 + j] over each cell's blocks into dst (a member of CaseScalars)
-void reduce_partials(const int* blk_start, int ncells, int ncase, const double* partial,
-                     CaseScalars* sc, int which, hipStream_t s);
+// (over the cells the launch lists, with the launch that wrote the partials)
+void reduce_partials(const Launch& L, int ncase, const double* partial, CaseScalars* sc, int which, hipStream_t s);
 enum { SC_FF = 0, SC_RZ_NEW, SC_PQ, SC_RR, SC_RZO, SC_RRS };
 // elementwise (cell-aligned 256-node blocks):
 template <typename T>
@@ -155,6 +155,9 @@ template <typename Td, typename Ts>
 void vec_copy(long long n, const Ts* src, Td* dst, hipStream_t s);
 template <typename T>
 void vec_zero(long long n, T* dst, hipStream_t s);
+// dst = src on the (cell, case) pairs a launch with convergence flags still iterates
+template <typename Td, typename Ts>
+void vec_copy_cases(const Launch& L, int nd, int ncase, const Ts* src, Td* dst, hipStream_t s);
 
 // ---- homogenization, sensitivities, filter, OC (homog.hip) -----------------
 // S_e packed upper triangle (nu = nc(nc+1)/2 values per element) and

@@ -7,11 +7,12 @@
 // per-block partials in a fixed order, then one wavefront per (cell, case).
 #include "kernels.h"
 
-__global__ __launch_bounds__(64) void k_reduce_partials(const int* __restrict__ blk_start, int ncase,
+__global__ __launch_bounds__(64) void k_reduce_partials(const LevelCell* __restrict__ cells,
+                                                        const int* __restrict__ blk_start, int ncase,
                                                         const double* __restrict__ partial,
                                                         CaseScalars* __restrict__ sc, int which) {
-  const int ci = blockIdx.x, j = blockIdx.y;
-  const int b0 = blk_start[ci], b1 = blk_start[ci + 1];
+  const int k = blockIdx.x, j = blockIdx.y, ci = cells[k].id;
+  const int b0 = blk_start[k], b1 = blk_start[k + 1];
   double v = 0.0;
   for (int b = b0 + (int)threadIdx.x; b < b1; b += 64) v += partial[(long long)b * MAPLE_MAX_CASES + j];
   v = wave_sum(v);
@@ -26,15 +27,16 @@ __global__ __launch_bounds__(64) void k_reduce_partials(const int* __restrict__ 
   }
 }
 
-void reduce_partials(const int* blk_start, int ncells, int ncase, const double* partial, CaseScalars* sc,
-                     int which, hipStream_t s) {
-  k_reduce_partials<<<dim3(ncells, ncase), 64, 0, s>>>(blk_start, ncase, partial, sc, which);
+void reduce_partials(const Launch& L, int ncase, const double* partial, CaseScalars* sc, int which, hipStream_t s) {
+  k_reduce_partials<<<dim3(L.ncells, ncase), 64, 0, s>>>(L.cells, L.blk_start, ncase, partial, sc, which);
 }
 
+// ci: the id of the block's cell (per-cell arrays); gid: the thread's node
 __device__ __forceinline__ bool vnode(const Launch& L, int& ci, long long& gid) {
-  ci = find_cell(L.blk_start, L.ncells, lbid());
-  const LevelCell c = L.cells[ci];
-  const long long n = (long long)(lbid() - L.blk_start[ci]) * 256 + threadIdx.x;
+  const int k = find_cell(L.blk_start, L.ncells, lbid());
+  const LevelCell c = L.cells[k];
+  ci = c.id;
+  const long long n = (long long)(lbid() - L.blk_start[k]) * 256 + threadIdx.x;
   if (n >= (long long)c.nx * c.ny * c.nz) return false;
   gid = c.off + n;
   return true;
@@ -63,6 +65,8 @@ void vec_scale_dinv(const Launch& L, int nd, int ncase, T omega, const T* dinv, 
 template void vec_scale_dinv<double>(const Launch&, int, int, double, const double*, const double*, double*, hipStream_t);
 template void vec_scale_dinv<float>(const Launch&, int, int, float, const float*, const float*, float*, hipStream_t);
 
+// A launch with convergence flags (L.done) skips converged (cell, case) pairs: their
+// partial is zero and their vectors are not read.
 template <typename Ta, typename Tb>
 __global__ __launch_bounds__(256) void k_dot(Launch L, int nd, int ncase, const Ta* __restrict__ a,
                                              const Tb* __restrict__ b, double* __restrict__ partial) {
@@ -70,7 +74,7 @@ __global__ __launch_bounds__(256) void k_dot(Launch L, int nd, int ncase, const 
   const bool ok = vnode(L, ci, gid);
   for (int j = 0; j < ncase; ++j) {
     double v[1] = {0.0};
-    if (ok)
+    if (ok && !case_done(L.done, L.ndc, ci, j))
       for (int i = 0; i < nd; ++i) {
         const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
         v[0] += (double)a[idx] * (double)b[idx];
@@ -246,6 +250,29 @@ void vec_copy(long long n, const Ts* src, Td* dst, hipStream_t s) {
 template void vec_copy<double, double>(long long, const double*, double*, hipStream_t);
 template void vec_copy<float, double>(long long, const double*, float*, hipStream_t);
 template void vec_copy<double, float>(long long, const float*, double*, hipStream_t);
+
+// dst = src over the (cell, case) pairs still iterating (all of them for a launch
+// without convergence flags)
+template <typename Td, typename Ts>
+__global__ __launch_bounds__(256) void k_copy_cases(Launch L, int nd, int ncase, const Ts* __restrict__ src,
+                                                    Td* __restrict__ dst) {
+  int ci; long long gid;
+  if (!vnode(L, ci, gid)) return;
+  for (int j = 0; j < ncase; ++j) {
+    if (case_done(L.done, L.ndc, ci, j)) continue;
+    for (int i = 0; i < nd; ++i) {
+      const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
+      dst[idx] = (Td)src[idx];
+    }
+  }
+}
+
+template <typename Td, typename Ts>
+void vec_copy_cases(const Launch& L, int nd, int ncase, const Ts* src, Td* dst, hipStream_t s) {
+  k_copy_cases<Td, Ts><<<L.nblocks, 256, 0, s>>>(L, nd, ncase, src, dst);
+}
+template void vec_copy_cases<float, double>(const Launch&, int, int, const double*, float*, hipStream_t);
+template void vec_copy_cases<double, float>(const Launch&, int, int, const float*, double*, hipStream_t);
 
 template <typename T>
 __global__ void k_zero(long long n, T* __restrict__ dst) {
