@@ -101,7 +101,7 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
         iters.append(info["iters"].max(axis=0).tolist())
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    count, kms = cells.timing_read(kind) if profile_kernel else (0, 0.0)
+    count, kms, work = cells.timing_read(kind, work=True) if profile_kernel else (0, 0.0, 0)
     cells.timing(False)
     # accuracy: the last step's C_H and kappa (solved at info["rho"]) against a tight FP64 solve
     rho = info["rho"]
@@ -125,13 +125,17 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
            "iters_th_mean": float(it[:, 6:].max(axis=1).mean()), "ch_rel_err": err, "J_last": float(info["J"][0]),
            "last_starts": src}
     if count:
+        # launches skip converged cells and load cases: the work of each launch is
+        # counted on the device (elements x load cases it processed)
         avg_s = kms / count * 1e-3
-        flops = ncells * 6 * R ** 3 * FLOP_PER_ELEM_ELASTIC
+        flops = work / count * FLOP_PER_ELEM_ELASTIC
         tname = "double" if precision == "fp64" else "float"
-        out["kernel"] = {"name": f"k_fine_apply<3,{tname},*> (level-0 elastic operator, 6 load cases x {ncells} cells)",
+        out["kernel"] = {"name": f"k_fine_apply<3,{tname},*> (level-0 elastic operator, up to 6 load cases x {ncells} "
+                                 f"cells per launch)",
                          "launches": count, "avg_us": avg_s * 1e6, "flop_per_launch": flops,
+                         "elem_cases_per_launch": work / count,
                          "achieved_tflops": flops / avg_s / 1e12, "peak": PEAK[precision],
-                         "traffic": ncells * TRAFFIC_PER_CELL[precision]}
+                         "traffic": work / count / (6 * R ** 3) * TRAFFIC_PER_CELL[precision]}
     return out
 
 

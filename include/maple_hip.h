@@ -106,7 +106,7 @@ int maple_set_concurrency(maple_ctx* ctx, int on);
 
 /* MAPLE_PREC_MIXED only: a solve that has not converged after `iters` CG
  * iterations builds the FP64 hierarchy of the same density and finishes on the
- * FP64 V-cycle (0 = never; default 100).  An FP32 V-cycle cannot resolve the
+ * FP64 V-cycle (0 = never; default 300).  An FP32 V-cycle cannot resolve the
  * soft modes of solid islands held by 1e-9 voids, where it stagnates (no
  * counterpart in the reference, whose solver runs in FP64 only). */
 int maple_set_mixed_fallback(maple_ctx* ctx, int iters);
@@ -119,8 +119,16 @@ int maple_set_mixed_fallback(maple_ctx* ctx, int iters);
 int maple_set_coarse_refresh(maple_ctx* ctx, int every);
 
 /* Kernel variants for the tuner (maple_amd/tune.py times them and keeps the
- * fastest): MAPLE_KERNEL_FINE_OPERATOR, variant 0 (all neighbour values held
- * in registers) or 1 (loads staged element by element, fewer live registers). */
+ * fastest): MAPLE_KERNEL_FINE_OPERATOR (the level-0 elastic operator of
+ * EBEOperator.matvec, EBE_GMG.py:412-430), variant 0 (element rows per pencil
+ * lane, all neighbour values held in registers), 1 (the same, loads staged
+ * element by element, fewer live registers), 2 (node blocks: each thread forms
+ * its node's 27 3x3 coupling blocks from the 8 element moduli around it and
+ * applies them to all six load cases), 3 (node blocks, three load cases per
+ * thread), 4 or 5 (node blocks as 2 or 3 for two x-adjacent nodes per thread,
+ * which gather their shared neighbours once), 6 or 7 (2x2 nodes in x and z per
+ * thread, three or two load cases).  Default: 5.  The thermal operator always
+ * uses the element-row kernel (variant 0, or 1 when 1 is set). */
 #define MAPLE_KERNEL_FINE_OPERATOR 0
 int maple_set_variant(maple_ctx* ctx, int kernel, int variant);
 
@@ -199,11 +207,14 @@ long long maple_num_elements(const maple_ctx* ctx);
 long long maple_num_nodes(const maple_ctx* ctx, int level);
 
 /* Kernel timing with HIP events on the call's stream around every launch of
- * the level-0 operator kernels (enable, then read: count of launches and their
- * summed duration in ms; reading resets).  kind 0: FP64 elastic, 1: FP64
- * thermal, 2: FP32 elastic (mixed form's preconditioner). */
+ * the level-0 operator kernels (enable, then read: count of launches, their
+ * summed duration in ms and, if `work` is not NULL, the elements x load cases
+ * those launches processed -- converged cells and cases are skipped, so a
+ * launch's work varies; reading resets).  kind 0: FP64 elastic, 1: FP64
+ * thermal, 2: FP32 elastic (mixed form's preconditioner).  No counterpart in
+ * the reference. */
 int maple_timing_enable(maple_ctx* ctx, int enable);
-int maple_timing_read(maple_ctx* ctx, int kind, long long* count, double* total_ms);
+int maple_timing_read(maple_ctx* ctx, int kind, long long* count, double* total_ms, long long* work);
 
 /* ---- learned warm start ---------------------------------------------------
  * A 3-level periodic 3-D U-Net (channels 16-32-64, 3x3x3 convolutions with

@@ -63,7 +63,7 @@ SIGNATURES = {
     "maple_num_elements": (C.c_longlong, [_P]),
     "maple_num_nodes": (C.c_longlong, [_P, _I]),
     "maple_timing_enable": (_I, [_P, _I]),
-    "maple_timing_read": (_I, [_P, _I, C.POINTER(C.c_longlong), _DP]),
+    "maple_timing_read": (_I, [_P, _I, C.POINTER(C.c_longlong), _DP, C.POINTER(C.c_longlong)]),
     "maple_unet_num_params": (C.c_longlong, []),
     "maple_unet_create": (_I, [_I, _I, _I, C.POINTER(_P)]),
     "maple_unet_destroy": (_I, [_P]),

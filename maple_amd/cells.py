@@ -125,7 +125,7 @@ class CellBatch:
         """Solve elastic and thermal at the same time (second stream and host thread) or in turn."""
         L.check(self.lib.maple_set_concurrency(self._h, int(bool(on))))
 
-    def set_mixed_fallback(self, iters: int = 100):
+    def set_mixed_fallback(self, iters: int = 300):
         """Mixed form: finish a solve on the FP64 V-cycle once it has run `iters` CG
         iterations (0 = never)."""
         L.check(self.lib.maple_set_mixed_fallback(self._h, int(iters)))
@@ -268,8 +268,13 @@ class CellBatch:
     def timing(self, enable: bool):
         L.check(self.lib.maple_timing_enable(self._h, int(bool(enable))))
 
-    def timing_read(self, kind: int = 0):
+    def timing_read(self, kind: int = 0, work: bool = False):
+        """(launches, total ms) of the timed level-0 operator launches since the last
+        read; with work=True also the elements x load cases they processed."""
         cnt = C.c_longlong(0)
         ms = C.c_double(0.0)
-        L.check(self.lib.maple_timing_read(self._h, int(kind), C.byref(cnt), C.byref(ms)))
+        wk = C.c_longlong(0)
+        L.check(self.lib.maple_timing_read(self._h, int(kind), C.byref(cnt), C.byref(ms), C.byref(wk)))
+        if work:
+            return int(cnt.value), float(ms.value), int(wk.value)
         return int(cnt.value), float(ms.value)

@@ -49,8 +49,10 @@ struct Level {
   int* dblk = nullptr;    // 256-node blocks
   int* dfblk = nullptr;   // level 0: operator blocks (4 waves)
   int* doblk = nullptr;   // level 0: 1024-element blocks (OC)
-  Launch node, fine, oc;
-  std::vector<int> cnt_node, cnt_fine;   // blocks per cell of the node and operator launches
+  int* dxblk = nullptr;   // level 0: 256 node pairs per block (2x1 node-block operator)
+  int* dqblk = nullptr;   // level 0: 256 node quads per block (2x2 node-block operator)
+  Launch node, fine, oc, pair, quad;
+  std::vector<int> cnt_node, cnt_fine, cnt_pair, cnt_quad;   // blocks per cell of the launches
 };
 
 // Multigrid hierarchy and V-cycle work vectors in precision T (FP64 for the
@@ -81,6 +83,7 @@ struct Phys {
   int* done_h = nullptr;   // pinned, 2 slots
   std::vector<Launch> ln;  // per-level node launches that skip converged (cell, case) pairs
   Launch lf;               // level-0 operator launch, same
+  Launch lx, lq;           // level-0 node-pair and node-quad operator launches, same
   double* atol2 = nullptr;
   bool have_fields = false, have_energy = false;
   double* ustart = nullptr;   // supplied start of the next solve
@@ -93,6 +96,10 @@ struct Phys {
   std::vector<int*> cb;
   LevelCell* ccf = nullptr;
   int* cbf = nullptr;
+  LevelCell* ccx = nullptr;
+  int* cbx = nullptr;
+  LevelCell* ccq = nullptr;
+  int* cbq = nullptr;
   char* stage = nullptr;
   size_t stage_bytes = 0;
   int stage_slots = 0, stage_next = 0, nlive = 0;
@@ -120,7 +127,7 @@ struct maple_ctx {
   int flexible = -1;   // -1: flexible CG in the mixed form only; 2: block CG
   int block_max = 200; // block CG iterations before per-case CG takes over
   int coarse_refresh = 1;   // rebuild the coarsest inverse every this many homogenizations
-  int fallback = 100;       // mixed form: FP64 V-cycle after this many CG iterations (0: never)
+  int fallback = 300;       // mixed form: FP64 V-cycle after this many CG iterations (0: never)
   OCState* ocst = nullptr;
   OCState* ocst_h = nullptr;
   double* oc_a = nullptr;
@@ -129,6 +136,7 @@ struct maple_ctx {
   bool concurrent = true;                      // solve elastic and thermal at the same time
   bool timing = false;
   Timer timers[3];   // 0: FP64 elastic level-0 operator, 1: FP64 thermal, 2: FP32 elastic
+  unsigned long long* work_d = nullptr;   // per timer: elements x load cases of the timed launches
   std::vector<void*> allocs;
   std::vector<void*> host_allocs;
 };
@@ -280,6 +288,12 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
   P.lf = c->lv[0].fine;
   P.lf.done = P.done_d;
   P.lf.ndc = nc;
+  P.lx = c->lv[0].pair;
+  P.lx.done = P.done_d;
+  P.lx.ndc = nc;
+  P.lq = c->lv[0].quad;
+  P.lq.done = P.done_d;
+  P.lq.ndc = nc;
   P.cc.assign(L, nullptr);
   P.cb.assign(L, nullptr);
   for (int l = 0; l < L; ++l) {
@@ -288,8 +302,12 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
   }
   TRY(dalloc(c, &P.ccf, (size_t)c->ncells));
   TRY(dalloc(c, &P.cbf, (size_t)c->ncells + 1));
+  TRY(dalloc(c, &P.ccx, (size_t)c->ncells));
+  TRY(dalloc(c, &P.cbx, (size_t)c->ncells + 1));
+  TRY(dalloc(c, &P.ccq, (size_t)c->ncells));
+  TRY(dalloc(c, &P.cbq, (size_t)c->ncells + 1));
   auto r8 = [](size_t b) { return (b + 7) / 8 * 8; };
-  P.stage_bytes = (size_t)(L + 1) * (r8(sizeof(LevelCell) * c->ncells) + r8(sizeof(int) * (c->ncells + 1)));
+  P.stage_bytes = (size_t)(L + 3) * (r8(sizeof(LevelCell) * c->ncells) + r8(sizeof(int) * (c->ncells + 1)));
   P.stage_slots = c->ncells + 2;   // a solve compacts at most ncells - 1 times: no slot is reused in flight
   void* sp = nullptr;
   HIPCHK(hipHostMalloc(&sp, P.stage_bytes * P.stage_slots, hipHostMallocDefault));
@@ -327,18 +345,36 @@ void timed_end(maple_ctx* c, int kind, hipStream_t s) {
   t.used += 2;
 }
 
+// Launch of the level-0 operator of physics P: the pencil launch of the element-row
+// kernels, or the node launch of the node-block kernels (elastic, variants 2-3).
+// full: every cell and case (no convergence flags).
+Launch op_launch(maple_ctx* c, const Phys& P, bool full) {
+  const int v = c->lv[0].fine.var;
+  const int var = P.nd == 3 ? v : (v == 1 ? 1 : 0);   // thermal: element rows only
+  Launch L = var >= 6 ? (full ? c->lv[0].quad : P.lq)
+           : var >= 4 ? (full ? c->lv[0].pair : P.lx)
+           : var >= 2 ? (full ? c->lv[0].node : P.ln[0])
+                      : (full ? c->lv[0].fine : P.lf);
+  L.var = var;
+  return L;
+}
+
 // FP64 level-0 operator for the CG matvec and residuals (any precision mode).
 void apply_fine64(maple_ctx* c, Phys& P, int mode, const double* rho, const double* u, const double* f, double* out,
                   double* partial, hipStream_t s) {
   const int kind = P.nd == 3 ? 0 : 1;
-  const Launch& L = mode == 0 ? c->lv[0].fine : P.lf;
+  Launch L = op_launch(c, P, mode == 0);
+  L.work = c->timing ? c->work_d + kind : nullptr;   // the launch counts its own work
+  const double* dinv = P.v64.dinv.empty() ? nullptr : P.v64.dinv[0];
   timed_begin(c, kind, s);
-  if (P.nd == 3)
-    fine_apply<3, double>(L, mode, P.nc, P.Ke, rho, u, f, P.v64.dinv.empty() ? nullptr : P.v64.dinv[0],
-                          c->omega, out, partial, s);
+  if (P.nd == 3 && L.var >= 4)
+    fine_apply_nodeb<double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
+  else if (P.nd == 3 && L.var >= 2)
+    fine_apply_node<double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
+  else if (P.nd == 3)
+    fine_apply<3, double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
   else
-    fine_apply<1, double>(L, mode, P.nc, P.Ke, rho, u, f, P.v64.dinv.empty() ? nullptr : P.v64.dinv[0],
-                          c->omega, out, partial, s);
+    fine_apply<1, double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
   timed_end(c, kind, s);
 }
 
@@ -347,11 +383,17 @@ template <typename T>
 void apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const T* u, const T* f, T* out, hipStream_t s) {
   if (l == 0) {
     const int kind = P.nd == 3 ? (sizeof(T) == 8 ? 0 : 2) : (sizeof(T) == 8 ? 1 : -1);
+    Launch L = op_launch(c, P, false);
+    L.work = c->timing && kind >= 0 ? c->work_d + kind : nullptr;
     if (kind >= 0) timed_begin(c, kind, s);
-    if (P.nd == 3)
-      fine_apply<3, T>(P.lf, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+    if (P.nd == 3 && L.var >= 4)
+      fine_apply_nodeb<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+    else if (P.nd == 3 && L.var >= 2)
+      fine_apply_node<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+    else if (P.nd == 3)
+      fine_apply<3, T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
     else
-      fine_apply<1, T>(P.lf, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+      fine_apply<1, T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
     if (kind >= 0) timed_end(c, kind, s);
   } else {
     if (P.nd == 3)
@@ -441,6 +483,12 @@ void full_launches(maple_ctx* c, Phys& P) {
   P.lf = c->lv[0].fine;
   P.lf.done = P.done_d;
   P.lf.ndc = P.nc;
+  P.lx = c->lv[0].pair;
+  P.lx.done = P.done_d;
+  P.lx.ndc = P.nc;
+  P.lq = c->lv[0].quad;
+  P.lq.done = P.done_d;
+  P.lq.ndc = P.nc;
   P.nlive = c->ncells;
 }
 
@@ -480,6 +528,8 @@ int compact_launches(maple_ctx* c, Phys& P, const int* done_h, hipStream_t s) {
   };
   for (size_t l = 0; l < P.ln.size(); ++l) TRY(build(c->lv[l], c->lv[l].cnt_node, P.cc[l], P.cb[l], P.ln[l]));
   TRY(build(c->lv[0], c->lv[0].cnt_fine, P.ccf, P.cbf, P.lf));
+  TRY(build(c->lv[0], c->lv[0].cnt_pair, P.ccx, P.cbx, P.lx));
+  TRY(build(c->lv[0], c->lv[0].cnt_quad, P.ccq, P.cbq, P.lq));
   P.nlive = n;
   return 0;
 }
@@ -537,7 +587,7 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   TRY(poll_done(c, P, 0, &all));
   for (int it = 0; it < maxit && !all; ++it) {
     apply_fine64(c, P, 1, rho, P.p, nullptr, P.q, P.partial, s);
-    reduce_partials(P.lf, P.nc, P.partial, P.sc, SC_PQ, s);
+    reduce_partials(op_launch(c, P, false), P.nc, P.partial, P.sc, SC_PQ, s);
     pcg_alpha(c->ncells, P.nc, P.sc, s);
     pcg_update_ur(P.ln[0], P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, P.partial, s);
     reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RR, s);
@@ -766,6 +816,9 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
     if (l == 0) {
       if ((rc = make_launch(c, L, 4, true, &L.dfblk, &L.fine, &L.cnt_fine))) return bail(rc);
       if ((rc = make_launch(c, L, 1024, false, &L.doblk, &L.oc))) return bail(rc);
+      if ((rc = make_launch(c, L, 512, false, &L.dxblk, &L.pair, &L.cnt_pair))) return bail(rc);
+      if ((rc = make_launch(c, L, 1024, false, &L.dqblk, &L.quad, &L.cnt_quad))) return bail(rc);
+      L.fine.var = 5;   // default level-0 elastic operator: 2x1 node blocks, three load cases per thread
     }
   }
   // partial-sum scratch: 80 doubles per block of the largest launch (block CG: two 6x6 Grams)
@@ -801,6 +854,9 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
       if (hipEventCreateWithFlags(&P.poll[k], hipEventDisableTiming) != hipSuccess)
         return bail(fail(MAPLE_EDEVICE, "event creation failed"));
   }
+  if ((rc = dalloc(c, &c->work_d, 3))) return bail(rc);
+  if (hipMemset(c->work_d, 0, 3 * sizeof(unsigned long long)) != hipSuccess)
+    return bail(fail(MAPLE_EDEVICE, "clearing the timing counters failed"));
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
@@ -858,7 +914,7 @@ int maple_set_solver(maple_ctx* c, double omega, int nsmooth, int warm_start) {
 int maple_set_variant(maple_ctx* c, int kernel, int variant) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
   if (kernel != MAPLE_KERNEL_FINE_OPERATOR) return fail(MAPLE_EINVAL, "unknown kernel id");
-  if (variant < 0 || variant > 1) return fail(MAPLE_EINVAL, "unknown variant");
+  if (variant < 0 || variant > 7) return fail(MAPLE_EINVAL, "unknown variant");
   c->lv[0].fine.var = variant;
   for (Phys& P : c->ph) P.lf.var = variant;
   return 0;
@@ -1104,9 +1160,10 @@ int maple_timing_enable(maple_ctx* c, int enable) {
   return 0;
 }
 
-int maple_timing_read(maple_ctx* c, int kind, long long* count, double* total_ms) {
+int maple_timing_read(maple_ctx* c, int kind, long long* count, double* total_ms, long long* work) {
   TRY(check_ctx(c));
   if (kind < 0 || kind > 2 || !count || !total_ms) return fail(MAPLE_EINVAL, "bad timing query");
+  TRY(set_device(c));
   Timer& t = c->timers[kind];
   double ms = 0.0;
   if (t.used > 0) HIPCHK(hipEventSynchronize(t.ev[t.used - 1]));
@@ -1118,6 +1175,11 @@ int maple_timing_read(maple_ctx* c, int kind, long long* count, double* total_ms
   *count = (long long)(t.used / 2);
   *total_ms = ms;
   t.used = 0;
+  HIPCHK(hipDeviceSynchronize());
+  unsigned long long w = 0;
+  HIPCHK(hipMemcpy(&w, c->work_d + kind, sizeof(w), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemset(c->work_d + kind, 0, sizeof(w)));
+  if (work) *work = (long long)w;
   return 0;
 }
 

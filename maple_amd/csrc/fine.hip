@@ -55,6 +55,26 @@ __device__ __forceinline__ LaneMap lane_map(const LevelCell& c, int wave, int la
 template <typename T>
 __device__ __forceinline__ T shfl_from(T v, int src) { return __shfl(v, src, 64); }
 
+// Work of one operator launch, for kernel timing: the first wavefront of block
+// (0, 0) adds the elements x load cases the launch processes (listed cells, cases
+// not converged when it runs) to *work.  The convergence flags do not change while
+// an operator launch runs, so every block of it sees the same ones.
+__device__ __forceinline__ void count_work(const LevelCell* __restrict__ cells, int ncells,
+                                           const int* __restrict__ done, int ndc, int ncase,
+                                           unsigned long long* __restrict__ work) {
+  if (work == nullptr || blockIdx.x != 0 || blockIdx.y != 0 || threadIdx.x >= 64) return;
+  unsigned long long w = 0;
+  for (int k = threadIdx.x; k < ncells; k += 64) {
+    const LevelCell c = cells[k];
+    const unsigned long long ne = (unsigned long long)c.nx * c.ny * c.nz;
+    for (int j = 0; j < ncase; ++j)
+      if (!(done != nullptr && done[c.id * ndc + j] != 0)) w += ne;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
+  if (threadIdx.x == 0) atomicAdd(work, w);
+}
+
 }  // namespace
 
 int fine_waves(const LevelCell& c) {
@@ -77,7 +97,9 @@ __global__ __launch_bounds__(256) void k_fine_apply(
     const LevelCell* __restrict__ cells, const int* __restrict__ blk_start, int ncells,
     long long Ntot, int ncase, const T* __restrict__ Ke, const T* __restrict__ rho,
     const T* __restrict__ u, const T* __restrict__ f, const T* __restrict__ dinv, T omega,
-    T* __restrict__ out, double* __restrict__ partial, const int* __restrict__ done, int ndc) {
+    T* __restrict__ out, double* __restrict__ partial, const int* __restrict__ done, int ndc,
+    unsigned long long* __restrict__ work) {
+  count_work(cells, ncells, done, ndc, ncase, work);
   constexpr int NE = 8 * ND;
   const int k = find_cell(blk_start, ncells, lbid());
   const LevelCell c = cells[k];
@@ -192,10 +214,10 @@ void fine_apply_v(const Launch& L, int mode, int ncase, const T* Ke, const T* rh
                   const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s) {
   dim3 grid(L.nblocks, ncase), block(256);
   switch (mode) {
-    case 0: k_fine_apply<ND, T, 0, VAR><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
-    case 1: k_fine_apply<ND, T, 1, VAR><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
-    case 2: k_fine_apply<ND, T, 2, VAR><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
-    default: k_fine_apply<ND, T, 3, VAR><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc); break;
+    case 0: k_fine_apply<ND, T, 0, VAR><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    case 1: k_fine_apply<ND, T, 1, VAR><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    case 2: k_fine_apply<ND, T, 2, VAR><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    default: k_fine_apply<ND, T, 3, VAR><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
   }
 }
 
@@ -311,3 +333,311 @@ void fine_dinv(const Launch& L, int nd, bool l1, const double* Ke, const double*
 }
 template void fine_dinv<double>(const Launch&, int, bool, const double*, const double*, double*, hipStream_t);
 template void fine_dinv<float>(const Launch&, int, bool, const double*, const double*, float*, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Node-block form of the level-0 elastic operator, every load case per lane.
+//
+//   (K u)_n = sum_{m in the 27-neighbourhood of n} K_nm u_m,
+//   K_nm    = sum_{elements e holding n and m} rho_e Ke[corner_e(n), corner_e(m)]  (3x3)
+//
+// The same sum as the element-row kernel, regrouped: each lane forms its node's
+// 27 blocks K_nm from the 8 element moduli around it (every entry of Ke is used
+// once: 576 FMA), then applies them to the load cases (243 FMA per case).  The
+// blocks are shared by the cases, so six cases cost 576 + 6*243 = 2034 FMA per
+// node instead of 6*576 = 3456.  One thread per node over the node launch
+// (cell-aligned blocks of 256 consecutive node ids: z-pencils, coalesced).
+// NCG: load cases per thread; blockIdx.y selects the group of NCG cases.
+template <typename T, int MODE, int NCG>
+__global__ __launch_bounds__(256) void k_fine_node(
+    const LevelCell* __restrict__ cells, const int* __restrict__ blk_start, int ncells,
+    long long Ntot, int ncase, const T* __restrict__ Ke, const T* __restrict__ rho,
+    const T* __restrict__ u, const T* __restrict__ f, const T* __restrict__ dinv, T omega,
+    T* __restrict__ out, double* __restrict__ partial, const int* __restrict__ done, int ndc,
+    unsigned long long* __restrict__ work) {
+  count_work(cells, ncells, done, ndc, ncase, work);
+  LevelCell c; int x, y, z; long long gid;
+  const bool valid = node_of(cells, blk_start, ncells, c, x, y, z, gid);
+  const int j0 = blockIdx.y * NCG;
+  bool live[NCG];
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < NCG; ++q) {
+    live[q] = (j0 + q < ncase) && !case_done(done, ndc, c.id, j0 + q);   // uniform over the block
+    any = any || live[q];
+  }
+  if (!any) return;   // every case of this group has converged: nothing is read again
+  Ke += (long long)c.id * 576;
+  if (!valid) { x = y = z = 0; gid = c.off; }   // in-range coordinates; the store is masked
+  const int xs[3] = {wrap(x - 1, c.nx), x, wrap(x + 1, c.nx)};
+  const int ys[3] = {wrap(y - 1, c.ny), y, wrap(y + 1, c.ny)};
+  const int zs[3] = {wrap(z - 1, c.nz), z, wrap(z + 1, c.nz)};
+  // moduli of the 8 elements (x-1+a, y-1+b, z-1+cc) around the node
+  T r[2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) r[a][b][cc] = rho[c.off + ((long long)xs[a] * c.ny + ys[b]) * c.nz + zs[cc]];
+
+  T acc[NCG][3], uown[NCG][3];
+#pragma unroll
+  for (int q = 0; q < NCG; ++q)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) acc[q][i] = uown[q][i] = T(0);
+
+#pragma unroll
+  for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int dz = -1; dz <= 1; ++dz) {
+        // K_nm: elements (a,b,cc) around n that also hold m = n + (dx,dy,dz)
+        T Kb[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) Kb[i][k] = T(0);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+              const int mx = 1 - a + dx, my = 1 - b + dy, mz = 1 - cc + dz;
+              if (mx < 0 || mx > 1 || my < 0 || my > 1 || mz < 0 || mz > 1) continue;
+              const int cn = hex_corner(1 - a, 1 - b, 1 - cc), cm = hex_corner(mx, my, mz);
+              const T re = r[a][b][cc];
+#pragma unroll
+              for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) Kb[i][k] = fma(re, Ke[(3 * cn + i) * 24 + 3 * cm + k], Kb[i][k]);
+            }
+        const long long m = c.off + ((long long)xs[dx + 1] * c.ny + ys[dy + 1]) * c.nz + zs[dz + 1];
+#pragma unroll
+        for (int q = 0; q < NCG; ++q) {
+          if (!live[q]) continue;
+          const T* __restrict__ uq = u + (long long)(j0 + q) * 3 * Ntot + m;
+          const T u0 = uq[0], u1 = uq[Ntot], u2 = uq[2 * Ntot];
+          if (dx == 0 && dy == 0 && dz == 0) {
+            uown[q][0] = u0; uown[q][1] = u1; uown[q][2] = u2;
+          }
+#pragma unroll
+          for (int i = 0; i < 3; ++i) acc[q][i] = fma(Kb[i][0], u0, fma(Kb[i][1], u1, fma(Kb[i][2], u2, acc[q][i])));
+        }
+      }
+
+  double dj[NCG];
+#pragma unroll
+  for (int q = 0; q < NCG; ++q) {
+    dj[q] = 0.0;
+    if (!live[q] || !valid) continue;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const long long idx = (long long)((j0 + q) * 3 + i) * Ntot + gid;
+      const T kv = acc[q][i];
+      if constexpr (MODE == 0 || MODE == 1) {
+        out[idx] = kv;
+        if constexpr (MODE == 1) dj[q] += (double)uown[q][i] * (double)kv;
+      } else if constexpr (MODE == 2) {
+        out[idx] = f[idx] - kv;
+      } else {
+        out[idx] = uown[q][i] + omega * dinv[(long long)i * Ntot + gid] * (f[idx] - kv);
+      }
+    }
+  }
+  if constexpr (MODE == 1) block_sum<NCG>(dj, partial + (long long)lbid() * MAPLE_MAX_CASES + j0);
+}
+
+template <typename T, int NCG>
+void fine_apply_node_g(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u, const T* f,
+                       const T* dinv, T omega, T* out, double* partial, hipStream_t s) {
+  dim3 grid(L.nblocks, (ncase + NCG - 1) / NCG), block(256);
+  switch (mode) {
+    case 0: k_fine_node<T, 0, NCG><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    case 1: k_fine_node<T, 1, NCG><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    case 2: k_fine_node<T, 2, NCG><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    default: k_fine_node<T, 3, NCG><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+  }
+}
+
+// variant 2: six cases per thread; variant 3: three cases per thread (twice the
+// blocks, the block assembly done twice)
+template <typename T>
+void fine_apply_node(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u, const T* f,
+                     const T* dinv, T omega, T* out, double* partial, hipStream_t s) {
+  if (L.var == 3) fine_apply_node_g<T, 3>(L, mode, ncase, Ke, rho, u, f, dinv, omega, out, partial, s);
+  else fine_apply_node_g<T, 6>(L, mode, ncase, Ke, rho, u, f, dinv, omega, out, partial, s);
+}
+template void fine_apply_node<double>(const Launch&, int, int, const double*, const double*, const double*, const double*, const double*, double, double*, double*, hipStream_t);
+template void fine_apply_node<float>(const Launch&, int, int, const float*, const float*, const float*, const float*, const float*, float, float*, double*, hipStream_t);
+
+// Node blocks, NX x NZ nodes per thread (variants 4-7).  Neighbouring nodes share
+// most of their 27-neighbourhoods, so each neighbour value is gathered once for
+// all the thread's nodes: (NX+2)*3*(NZ+2) gathers of every case's components for
+// NX*NZ nodes, against 27 per node one node at a time (2x1: 18 per node, 2x2: 12).
+// The launch has blocks of 256 threads; thread t of a cell owns the nodes
+// (NX*gx + kx, y, NZ*gz + kz), t = (gx*ny + y)*(nz/NZ) + gz (nx, nz even).  The
+// block assembly stays per node: 576 FMA per node and case group.
+template <typename T, int MODE, int NCG, int NX, int NZ>
+__global__ __launch_bounds__(256) void k_fine_nodeb(
+    const LevelCell* __restrict__ cells, const int* __restrict__ blk_start, int ncells,
+    long long Ntot, int ncase, const T* __restrict__ Ke, const T* __restrict__ rho,
+    const T* __restrict__ u, const T* __restrict__ f, const T* __restrict__ dinv, T omega,
+    T* __restrict__ out, double* __restrict__ partial, const int* __restrict__ done, int ndc,
+    unsigned long long* __restrict__ work) {
+  count_work(cells, ncells, done, ndc, ncase, work);
+  const int k = find_cell(blk_start, ncells, lbid());
+  const LevelCell c = cells[k];
+  const int j0 = blockIdx.y * NCG;
+  bool live[NCG];
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < NCG; ++q) {
+    live[q] = (j0 + q < ncase) && !case_done(done, ndc, c.id, j0 + q);   // uniform over the block
+    any = any || live[q];
+  }
+  if (!any) return;
+  Ke += (long long)c.id * 576;
+  const int nzt = c.nz / NZ;
+  const long long t = (long long)(lbid() - blk_start[k]) * 256 + threadIdx.x;
+  const bool valid = t < (long long)(c.nx / NX) * c.ny * nzt;
+  int x0 = 0, y = 0, z0 = 0;
+  if (valid) {
+    z0 = (int)(t % nzt) * NZ;
+    const long long tt = t / nzt;
+    y = (int)(tt % c.ny);
+    x0 = (int)(tt / c.ny) * NX;
+  }
+  int xs[NX + 2], zs[NZ + 2];
+#pragma unroll
+  for (int i = 0; i < NX + 2; ++i) xs[i] = wrap(x0 - 1 + i, c.nx);
+#pragma unroll
+  for (int i = 0; i < NZ + 2; ++i) zs[i] = wrap(z0 - 1 + i, c.nz);
+  const int ys[3] = {wrap(y - 1, c.ny), y, wrap(y + 1, c.ny)};
+  // moduli of the elements (x0-1+a, y-1+b, z0-1+cc)
+  T r[NX + 1][2][NZ + 1];
+#pragma unroll
+  for (int a = 0; a < NX + 1; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int cc = 0; cc < NZ + 1; ++cc) r[a][b][cc] = rho[c.off + ((long long)xs[a] * c.ny + ys[b]) * c.nz + zs[cc]];
+
+  T acc[NX][NZ][NCG][3];
+#pragma unroll
+  for (int kx = 0; kx < NX; ++kx)
+#pragma unroll
+    for (int kz = 0; kz < NZ; ++kz)
+#pragma unroll
+      for (int q = 0; q < NCG; ++q)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc[kx][kz][q][i] = T(0);
+
+#pragma unroll
+  for (int ix = 0; ix < NX + 2; ++ix)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int iz = 0; iz < NZ + 2; ++iz) {
+        const long long m = c.off + ((long long)xs[ix] * c.ny + ys[dy + 1]) * c.nz + zs[iz];
+        T uv[NCG][3];
+#pragma unroll
+        for (int q = 0; q < NCG; ++q) {
+          if (!live[q]) continue;
+          const T* __restrict__ uq = u + (long long)(j0 + q) * 3 * Ntot + m;
+          uv[q][0] = uq[0]; uv[q][1] = uq[Ntot]; uv[q][2] = uq[2 * Ntot];
+        }
+#pragma unroll
+        for (int kx = 0; kx < NX; ++kx)
+#pragma unroll
+          for (int kz = 0; kz < NZ; ++kz) {
+            const int dx = ix - 1 - kx, dz = iz - 1 - kz;   // neighbour offset from own node (kx, kz)
+            if (dx < -1 || dx > 1 || dz < -1 || dz > 1) continue;
+            T Kb[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int kk = 0; kk < 3; ++kk) Kb[i][kk] = T(0);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+              for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) {
+                  const int mx = 1 - a + dx, my = 1 - b + dy, mz = 1 - cc + dz;
+                  if (mx < 0 || mx > 1 || my < 0 || my > 1 || mz < 0 || mz > 1) continue;
+                  const int cn = hex_corner(1 - a, 1 - b, 1 - cc), cm = hex_corner(mx, my, mz);
+                  const T re = r[kx + a][b][kz + cc];
+#pragma unroll
+                  for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int kk = 0; kk < 3; ++kk) Kb[i][kk] = fma(re, Ke[(3 * cn + i) * 24 + 3 * cm + kk], Kb[i][kk]);
+                }
+#pragma unroll
+            for (int q = 0; q < NCG; ++q) {
+              if (!live[q]) continue;
+#pragma unroll
+              for (int i = 0; i < 3; ++i)
+                acc[kx][kz][q][i] =
+                    fma(Kb[i][0], uv[q][0], fma(Kb[i][1], uv[q][1], fma(Kb[i][2], uv[q][2], acc[kx][kz][q][i])));
+            }
+          }
+      }
+
+  double dj[NCG];
+#pragma unroll
+  for (int q = 0; q < NCG; ++q) dj[q] = 0.0;
+  if (valid) {
+#pragma unroll
+    for (int kx = 0; kx < NX; ++kx)
+#pragma unroll
+      for (int kz = 0; kz < NZ; ++kz) {
+        const long long gid = c.off + ((long long)(x0 + kx) * c.ny + y) * c.nz + z0 + kz;
+#pragma unroll
+        for (int q = 0; q < NCG; ++q) {
+          if (!live[q]) continue;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const long long idx = (long long)((j0 + q) * 3 + i) * Ntot + gid;
+            const T kv = acc[kx][kz][q][i];
+            if constexpr (MODE == 0 || MODE == 1) {
+              out[idx] = kv;
+              if constexpr (MODE == 1) dj[q] += (double)u[idx] * (double)kv;
+            } else if constexpr (MODE == 2) {
+              out[idx] = f[idx] - kv;
+            } else {
+              out[idx] = u[idx] + omega * dinv[(long long)i * Ntot + gid] * (f[idx] - kv);
+            }
+          }
+        }
+      }
+  }
+  if constexpr (MODE == 1) block_sum<NCG>(dj, partial + (long long)lbid() * MAPLE_MAX_CASES + j0);
+}
+
+template <typename T, int NCG, int NX, int NZ>
+void fine_apply_nodeb_g(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u, const T* f,
+                        const T* dinv, T omega, T* out, double* partial, hipStream_t s) {
+  dim3 grid(L.nblocks, (ncase + NCG - 1) / NCG), block(256);
+  switch (mode) {
+    case 0: k_fine_nodeb<T, 0, NCG, NX, NZ><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    case 1: k_fine_nodeb<T, 1, NCG, NX, NZ><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    case 2: k_fine_nodeb<T, 2, NCG, NX, NZ><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    default: k_fine_nodeb<T, 3, NCG, NX, NZ><<<grid, block, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, Ke, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+  }
+}
+
+// variants 4, 5: 2x1 nodes, six / three cases per thread; 6, 7: 2x2 nodes, three / two
+template <typename T>
+void fine_apply_nodeb(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u, const T* f,
+                      const T* dinv, T omega, T* out, double* partial, hipStream_t s) {
+  switch (L.var) {
+    case 4: fine_apply_nodeb_g<T, 6, 2, 1>(L, mode, ncase, Ke, rho, u, f, dinv, omega, out, partial, s); break;
+    case 5: fine_apply_nodeb_g<T, 3, 2, 1>(L, mode, ncase, Ke, rho, u, f, dinv, omega, out, partial, s); break;
+    case 6: fine_apply_nodeb_g<T, 3, 2, 2>(L, mode, ncase, Ke, rho, u, f, dinv, omega, out, partial, s); break;
+    default: fine_apply_nodeb_g<T, 2, 2, 2>(L, mode, ncase, Ke, rho, u, f, dinv, omega, out, partial, s); break;
+  }
+}
+template void fine_apply_nodeb<double>(const Launch&, int, int, const double*, const double*, const double*, const double*, const double*, double, double*, double*, hipStream_t);
+template void fine_apply_nodeb<float>(const Launch&, int, int, const float*, const float*, const float*, const float*, const float*, float, float*, double*, hipStream_t);

@@ -18,7 +18,14 @@ struct Launch {
   // still iterating
   const int* done = nullptr;
   int ndc = 0;
-  int var = 0;   // kernel variant (level-0 operator: register staging), picked by the tuner
+  // kernel variant of the level-0 elastic operator, picked by the tuner: element rows
+  // per pencil lane (0: neighbour values in registers, 1: staged element by element)
+  // or node blocks (2: six load cases per thread, 3: three; 4, 5: the same with two
+  // x-adjacent nodes per thread; 6, 7: 2x2 nodes in x and z, three / two cases)
+  int var = 0;
+  // level-0 operator timing: when set, the launch adds the elements x load cases it
+  // processes (listed cells, cases not converged) to *work
+  unsigned long long* work = nullptr;
 };
 
 __device__ __forceinline__ bool case_done(const int* done, int ndc, int ci, int j) {
@@ -42,6 +49,16 @@ int fine_waves(const LevelCell& c);  // wavefronts per cell of k_fine_apply (4 p
 template <int ND, typename T>
 void fine_apply(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u,
                 const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s);
+// Node-block form of the elastic operator (variants 2 and 3), same modes; L is the
+// node launch (256 consecutive nodes per block).
+template <typename T>
+void fine_apply_node(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u,
+                     const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s);
+// Node blocks, 2x1 (variants 4, 5) or 2x2 (6, 7) nodes per thread; L: blocks of 256
+// threads over the cell's nodes / (nodes per thread) threads.
+template <typename T>
+void fine_apply_nodeb(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u,
+                      const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s);
 void fine_rhs(const Launch& L, int nd, int ncase, const double* Fe, const double* rho, double* F, hipStream_t s);
 template <typename T>
 void fine_dinv(const Launch& L, int nd, bool l1, const double* Ke, const double* rho, T* dinv, hipStream_t s);

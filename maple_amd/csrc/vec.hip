@@ -42,6 +42,28 @@ __device__ __forceinline__ bool vnode(const Launch& L, int& ci, long long& gid) 
   return true;
 }
 
+// The vector kernels come in two forms: ND and NC (components per node, load
+// cases) as compile-time constants for the two physics (3 x 6 elastic, 1 x 3
+// thermal), fully unrolled so that every load of a thread is in flight at once
+// and a dot product needs one block reduction for all its cases; and a generic
+// form for any other shape.
+template <int ND, int NC, typename T>
+__global__ __launch_bounds__(256) void k_scale_dinv_t(Launch L, T omega, const T* __restrict__ dinv,
+                                                      const T* __restrict__ b, T* __restrict__ x) {
+  int ci; long long gid;
+  if (!vnode(L, ci, gid)) return;
+  if (cell_done(L.done, L.ndc, ci)) return;
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    const T w = omega * dinv[(long long)i * L.Ntot + gid];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
+      x[idx] = w * b[idx];
+    }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_scale_dinv(Launch L, int nd, int ncase, T omega,
                                                     const T* __restrict__ dinv, const T* __restrict__ b,
@@ -60,13 +82,34 @@ __global__ __launch_bounds__(256) void k_scale_dinv(Launch L, int nd, int ncase,
 
 template <typename T>
 void vec_scale_dinv(const Launch& L, int nd, int ncase, T omega, const T* dinv, const T* b, T* x, hipStream_t s) {
-  k_scale_dinv<T><<<L.nblocks, 256, 0, s>>>(L, nd, ncase, omega, dinv, b, x);
+  if (nd == 3 && ncase == 6) k_scale_dinv_t<3, 6, T><<<L.nblocks, 256, 0, s>>>(L, omega, dinv, b, x);
+  else if (nd == 1 && ncase == 3) k_scale_dinv_t<1, 3, T><<<L.nblocks, 256, 0, s>>>(L, omega, dinv, b, x);
+  else k_scale_dinv<T><<<L.nblocks, 256, 0, s>>>(L, nd, ncase, omega, dinv, b, x);
 }
 template void vec_scale_dinv<double>(const Launch&, int, int, double, const double*, const double*, double*, hipStream_t);
 template void vec_scale_dinv<float>(const Launch&, int, int, float, const float*, const float*, float*, hipStream_t);
 
 // A launch with convergence flags (L.done) skips converged (cell, case) pairs: their
 // partial is zero and their vectors are not read.
+template <int ND, int NC, typename Ta, typename Tb>
+__global__ __launch_bounds__(256) void k_dot_t(Launch L, const Ta* __restrict__ a, const Tb* __restrict__ b,
+                                               double* __restrict__ partial) {
+  int ci; long long gid = 0;
+  const bool ok = vnode(L, ci, gid);
+  double v[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    v[j] = 0.0;
+    if (ok && !case_done(L.done, L.ndc, ci, j))
+#pragma unroll
+      for (int i = 0; i < ND; ++i) {
+        const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
+        v[j] += (double)a[idx] * (double)b[idx];
+      }
+  }
+  block_sum<NC>(v, partial + (long long)lbid() * MAPLE_MAX_CASES);
+}
+
 template <typename Ta, typename Tb>
 __global__ __launch_bounds__(256) void k_dot(Launch L, int nd, int ncase, const Ta* __restrict__ a,
                                              const Tb* __restrict__ b, double* __restrict__ partial) {
@@ -85,11 +128,39 @@ __global__ __launch_bounds__(256) void k_dot(Launch L, int nd, int ncase, const 
 
 template <typename Ta, typename Tb>
 void vec_dot(const Launch& L, int nd, int ncase, const Ta* a, const Tb* b, double* partial, hipStream_t s) {
-  k_dot<Ta, Tb><<<L.nblocks, 256, 0, s>>>(L, nd, ncase, a, b, partial);
+  if (nd == 3 && ncase == 6) k_dot_t<3, 6, Ta, Tb><<<L.nblocks, 256, 0, s>>>(L, a, b, partial);
+  else if (nd == 1 && ncase == 3) k_dot_t<1, 3, Ta, Tb><<<L.nblocks, 256, 0, s>>>(L, a, b, partial);
+  else k_dot<Ta, Tb><<<L.nblocks, 256, 0, s>>>(L, nd, ncase, a, b, partial);
 }
 template void vec_dot<double, double>(const Launch&, int, int, const double*, const double*, double*, hipStream_t);
 template void vec_dot<double, float>(const Launch&, int, int, const double*, const float*, double*, hipStream_t);
 template void vec_dot<float, float>(const Launch&, int, int, const float*, const float*, double*, hipStream_t);
+
+template <int ND, int NC>
+__global__ __launch_bounds__(256) void k_update_ur_t(Launch L, const CaseScalars* __restrict__ sc,
+                                                     const double* __restrict__ p, const double* __restrict__ q,
+                                                     double* __restrict__ u, double* __restrict__ r,
+                                                     double* __restrict__ partial) {
+  int ci; long long gid = 0;
+  const bool ok = vnode(L, ci, gid);
+  double v[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    v[j] = 0.0;
+    const double al = sc[ci * NC + j].alpha;
+    if (ok && al != 0.0) {   // a converged case keeps u and r untouched (its p may be stale)
+#pragma unroll
+      for (int i = 0; i < ND; ++i) {
+        const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
+        u[idx] = fma(al, p[idx], u[idx]);
+        const double rn = fma(-al, q[idx], r[idx]);
+        r[idx] = rn;
+        v[j] += rn * rn;
+      }
+    }
+  }
+  block_sum<NC>(v, partial + (long long)lbid() * MAPLE_MAX_CASES);
+}
 
 __global__ __launch_bounds__(256) void k_update_ur(Launch L, int nd, int ncase, const CaseScalars* __restrict__ sc,
                                                    const double* __restrict__ p, const double* __restrict__ q,
@@ -100,7 +171,7 @@ __global__ __launch_bounds__(256) void k_update_ur(Launch L, int nd, int ncase, 
   for (int j = 0; j < ncase; ++j) {
     double v[1] = {0.0};
     const double al = ok ? sc[ci * ncase + j].alpha : 0.0;
-    if (ok && al != 0.0) {   // a converged case keeps u and r untouched (its p may be stale)
+    if (ok && al != 0.0) {
       for (int i = 0; i < nd; ++i) {
         const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
         u[idx] = fma(al, p[idx], u[idx]);
@@ -115,7 +186,26 @@ __global__ __launch_bounds__(256) void k_update_ur(Launch L, int nd, int ncase, 
 
 void pcg_update_ur(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* p, const double* q,
                    double* u, double* r, double* partial, hipStream_t s) {
-  k_update_ur<<<L.nblocks, 256, 0, s>>>(L, nd, ncase, sc, p, q, u, r, partial);
+  if (nd == 3 && ncase == 6) k_update_ur_t<3, 6><<<L.nblocks, 256, 0, s>>>(L, sc, p, q, u, r, partial);
+  else if (nd == 1 && ncase == 3) k_update_ur_t<1, 3><<<L.nblocks, 256, 0, s>>>(L, sc, p, q, u, r, partial);
+  else k_update_ur<<<L.nblocks, 256, 0, s>>>(L, nd, ncase, sc, p, q, u, r, partial);
+}
+
+template <int ND, int NC>
+__global__ __launch_bounds__(256) void k_update_p_t(Launch L, const CaseScalars* __restrict__ sc,
+                                                    const double* __restrict__ z, double* __restrict__ p) {
+  int ci; long long gid;
+  if (!vnode(L, ci, gid)) return;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    if (sc[ci * NC + j].done) continue;   // a converged case's p is never used again
+    const double be = sc[ci * NC + j].beta;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
+      p[idx] = fma(be, p[idx], z[idx]);
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_update_p(Launch L, int nd, int ncase, const CaseScalars* __restrict__ sc,
@@ -123,7 +213,7 @@ __global__ __launch_bounds__(256) void k_update_p(Launch L, int nd, int ncase, c
   int ci; long long gid;
   if (!vnode(L, ci, gid)) return;
   for (int j = 0; j < ncase; ++j) {
-    if (sc[ci * ncase + j].done) continue;   // a converged case's p is never used again
+    if (sc[ci * ncase + j].done) continue;
     const double be = sc[ci * ncase + j].beta;
     for (int i = 0; i < nd; ++i) {
       const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
@@ -134,7 +224,9 @@ __global__ __launch_bounds__(256) void k_update_p(Launch L, int nd, int ncase, c
 
 void pcg_update_p(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* z, double* p,
                   hipStream_t s) {
-  k_update_p<<<L.nblocks, 256, 0, s>>>(L, nd, ncase, sc, z, p);
+  if (nd == 3 && ncase == 6) k_update_p_t<3, 6><<<L.nblocks, 256, 0, s>>>(L, sc, z, p);
+  else if (nd == 1 && ncase == 3) k_update_p_t<1, 3><<<L.nblocks, 256, 0, s>>>(L, sc, z, p);
+  else k_update_p<<<L.nblocks, 256, 0, s>>>(L, nd, ncase, sc, z, p);
 }
 
 __global__ void k_pick_src(int n, int have_prev, int have_start, CaseScalars* __restrict__ sc) {
@@ -253,6 +345,21 @@ template void vec_copy<double, float>(long long, const float*, double*, hipStrea
 
 // dst = src over the (cell, case) pairs still iterating (all of them for a launch
 // without convergence flags)
+template <int ND, int NC, typename Td, typename Ts>
+__global__ __launch_bounds__(256) void k_copy_cases_t(Launch L, const Ts* __restrict__ src, Td* __restrict__ dst) {
+  int ci; long long gid;
+  if (!vnode(L, ci, gid)) return;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    if (case_done(L.done, L.ndc, ci, j)) continue;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
+      dst[idx] = (Td)src[idx];
+    }
+  }
+}
+
 template <typename Td, typename Ts>
 __global__ __launch_bounds__(256) void k_copy_cases(Launch L, int nd, int ncase, const Ts* __restrict__ src,
                                                     Td* __restrict__ dst) {
@@ -269,7 +376,9 @@ __global__ __launch_bounds__(256) void k_copy_cases(Launch L, int nd, int ncase,
 
 template <typename Td, typename Ts>
 void vec_copy_cases(const Launch& L, int nd, int ncase, const Ts* src, Td* dst, hipStream_t s) {
-  k_copy_cases<Td, Ts><<<L.nblocks, 256, 0, s>>>(L, nd, ncase, src, dst);
+  if (nd == 3 && ncase == 6) k_copy_cases_t<3, 6, Td, Ts><<<L.nblocks, 256, 0, s>>>(L, src, dst);
+  else if (nd == 1 && ncase == 3) k_copy_cases_t<1, 3, Td, Ts><<<L.nblocks, 256, 0, s>>>(L, src, dst);
+  else k_copy_cases<Td, Ts><<<L.nblocks, 256, 0, s>>>(L, nd, ncase, src, dst);
 }
 template void vec_copy_cases<float, double>(const Launch&, int, int, const double*, float*, hipStream_t);
 template void vec_copy_cases<double, float>(const Launch&, int, int, const float*, double*, hipStream_t);

@@ -4,7 +4,8 @@ For every tunable kernel the tuner times each variant on the batch's own
 shapes (the real cell sizes, all load cases, HIP events on the batch's
 stream) and keeps the fastest.  The choice depends on the cell mix: the
 register-staged level-0 operator wins when occupancy is the limit, the
-all-in-registers one when L1 re-loads are.
+all-in-registers one when L1 re-loads are; the node-block forms (six or three
+load cases per thread) when the arithmetic is.
 """
 from __future__ import annotations
 
@@ -13,7 +14,7 @@ import torch
 from . import _lib as L
 from .cells import CellBatch
 
-KERNELS = {"fine_operator": (L.MAPLE_KERNEL_FINE_OPERATOR, (0, 1))}
+KERNELS = {"fine_operator": (L.MAPLE_KERNEL_FINE_OPERATOR, (0, 1, 2, 3, 4, 5, 6, 7))}
 
 
 def _time(fn, reps: int) -> float:

@@ -51,6 +51,47 @@ def test_operator_and_rhs(r, nlev, physics):
     assert np.abs(F - Fo.transpose(2, 1, 0)).max() <= 1e-13 * np.abs(Fo).max()
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("r,nlev,ncase", [(16, 3, 6), (24, 3, 4), (72, 4, 5)])
+def test_operator_variants(r, nlev, ncase, variant):
+    """Every level-0 elastic operator variant (element rows per pencil lane, staged;
+    node blocks with six or three load cases per thread, one, 2x1 or 2x2 nodes
+    per thread) against the oracle's EBE product, also for fewer load cases than six."""
+    import maple_amd
+    print(f"[operator variant] r={r} variant={variant} ncase={ncase}", flush=True)
+    cb = _cells([r], nlev)
+    cb.set_variant(maple_amd._lib.MAPLE_KERNEL_FINE_OPERATOR, variant)
+    rho = _rho(r, r + variant)
+    rng = np.random.default_rng(variant)
+    u = rng.standard_normal((ncase, 3, r ** 3))
+    v = cb.apply_operator("elastic", _t(rho), _t(u)).cpu().numpy()
+    Ke = ofe.hexahedron(r, ofe.isotropic_elastic_tensor(1, 0.3))[0]
+    for j in range(ncase):
+        ref = ogrid.ebe_matvec(u[j].T.copy(), rho, Ke, (r, r, r)).T
+        assert np.abs(v[j] - ref).max() <= 1e-13 * np.abs(ref).max()
+    cb.close()
+
+
+@pytest.mark.parametrize("variant", [2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_homogenize_node_variants(variant, precision):
+    """The node-block operator inside the whole solve (CG matvec, residuals, and the
+    FP32 Jacobi sweeps of the mixed form's preconditioner) against the oracle."""
+    import maple_amd
+    r, nlev = 16, 3
+    print(f"[homogenize variant] variant={variant} {precision}", flush=True)
+    cb = maple_amd.CellBatch([r], nlevels=nlev, device=DEV, precision=precision)
+    cb.set_variant(maple_amd._lib.MAPLE_KERNEL_FINE_OPERATOR, variant)
+    rho = _rho(r, 11)
+    CH, kap, it = cb.homogenize(_t(rho), rtol=1e-10)
+    cs = ohomog.CellSolver((r, r, r), nlevels=3, rtol=1e-12)
+    CHo, _, _ = cs.elastic(rho)
+    ko, _, _ = cs.thermal(rho)
+    assert np.linalg.norm(CH[0].cpu().numpy() - CHo) <= 1e-9 * np.linalg.norm(CHo)
+    assert np.linalg.norm(kap[0].cpu().numpy() - ko) <= 1e-9 * np.linalg.norm(ko)
+    cb.close()
+
+
 @pytest.mark.parametrize("r,nlev,seed", [(8, 2, 0), (8, 3, 1), (16, 3, 2), (16, 4, 3)])
 def test_homogenize_parity(r, nlev, seed):
     print(f"[homogenize] r={r} nlev={nlev}", flush=True)
