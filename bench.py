@@ -12,15 +12,17 @@ disconnected islands slow the cell solves down.
 Forms (one JSON line; ``value`` is the headline form):
 * ``plain``    one cell, FP64, zero start every solve: the reference's method on the GPU.
 * ``headline`` 16 cells advanced together (one set of launches per OC step),
-               FP64 CG around an FP32 multigrid preconditioner, each solve
-               starting from the best of zero, the previous iterate's fields
-               and the learned network's prediction.
+               FP64 CG around an FP32 multigrid preconditioner with the FP64
+               island correction (small solid islands solved exactly beside the
+               V-cycle), each solve starting from the best of zero, the previous
+               iterate's fields and the learned network's prediction.
                value = cells x iterations / s.
 * optional (--forms): single-cell and batched ablations (``fp64-warm``,
-  ``fp64-block``, ``mixed-warm``, ``mixed-warm-block``, ``mixed-warm-learned``,
-  ``batched-mixed-warm``, ``batched-mixed-warm-block``, ``headline-block`` with
-  block CG over the load cases); --config4 adds the 64-cell mixed-resolution
-  batch (64^3-equivalent iterations / s).
+  ``fp64-islands``, ``fp64-block``, ``mixed-warm``, ``mixed-warm-block``,
+  ``mixed-warm-learned``, ``batched-mixed-warm``, ``batched-mixed-warm-islands``,
+  ``batched-mixed-warm-block``, ``headline-block`` with block CG over the load
+  cases); --config4 adds the 64-cell mixed-resolution batch (64^3-equivalent
+  iterations / s).
 
 Accuracy: the last iterate's C_H and kappa of the headline form (every cell)
 against a tight (rtol 1e-11) FP64 solve of the same densities.  ``roofline`` is
@@ -60,7 +62,8 @@ def _x0(ncells):
 
 
 def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", warm: bool = False,
-             learned: bool = False, block: bool = False, maxit: int = 5000, profile_kernel: bool = True):
+             learned: bool = False, block: bool = False, islands: bool = False, maxit: int = 5000,
+             profile_kernel: bool = True):
     import numpy as np
     import torch
 
@@ -71,6 +74,8 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
     def make():
         cells = maple_amd.CellBatch([R] * ncells, nlevels=NLEVELS, device=dev, warm_start=warm, precision=precision,
                                     block_cg=block)
+        if islands:
+            cells.set_islands(True)
         start = None
         if learned:
             from maple_amd import learned as ml
@@ -150,6 +155,7 @@ def run_config4(steps: int, precision: str = "mixed", maxit: int = 5000):
     spec = maple_amd.batched_workload()
     cells = maple_amd.CellBatch([c["r"] for c in spec], nlevels=NLEVELS, device=dev, precision=precision,
                                 warm_start=True)
+    cells.set_islands(True)
     drv = maple_amd.OCDriver(cells, [c["weights"] for c in spec], [c["volfrac"] for c in spec], move=0.2, xmin=1e-3,
                              filter_k=3, rtol=1e-6, maxit=maxit)
     x = torch.from_numpy(np.concatenate([c["x0"] for c in spec])).to(dev)
@@ -182,16 +188,20 @@ def cpu_baseline(n_iter: int):
 
 
 FORMS = {
-    "plain": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False),
-    "headline": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False),
-    "headline-block": dict(ncells=16, precision="mixed", warm=True, learned=True, block=True),
-    "fp64-warm": dict(ncells=1, precision="fp64", warm=True, learned=False, block=False),
-    "fp64-block": dict(ncells=1, precision="fp64", warm=False, learned=False, block=True),
-    "mixed-warm": dict(ncells=1, precision="mixed", warm=True, learned=False, block=False),
-    "mixed-warm-block": dict(ncells=1, precision="mixed", warm=True, learned=False, block=True),
-    "mixed-warm-learned": dict(ncells=1, precision="mixed", warm=True, learned=True, block=False),
-    "batched-mixed-warm": dict(ncells=16, precision="mixed", warm=True, learned=False, block=False),
-    "batched-mixed-warm-block": dict(ncells=16, precision="mixed", warm=True, learned=False, block=True),
+    "plain": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False, islands=False),
+    "headline": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True),
+    "headline-block": dict(ncells=16, precision="mixed", warm=True, learned=True, block=True, islands=True),
+    "fp64-warm": dict(ncells=1, precision="fp64", warm=True, learned=False, block=False, islands=False),
+    "fp64-islands": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False, islands=True),
+    "fp64-block": dict(ncells=1, precision="fp64", warm=False, learned=False, block=True, islands=False),
+    "mixed-warm": dict(ncells=1, precision="mixed", warm=True, learned=False, block=False, islands=False),
+    "mixed-warm-block": dict(ncells=1, precision="mixed", warm=True, learned=False, block=True, islands=False),
+    "mixed-warm-learned": dict(ncells=1, precision="mixed", warm=True, learned=True, block=False, islands=False),
+    "batched-mixed-warm": dict(ncells=16, precision="mixed", warm=True, learned=False, block=False, islands=False),
+    "batched-mixed-warm-islands": dict(ncells=16, precision="mixed", warm=True, learned=False, block=False,
+                                       islands=True),
+    "batched-mixed-warm-block": dict(ncells=16, precision="mixed", warm=True, learned=False, block=True,
+                                     islands=False),
 }
 
 
@@ -240,6 +250,7 @@ def main():
                                "filter k=3, 4-level MG-PCG rtol 1e-6",
                    "form": head, "cells": spec["ncells"], "precision": spec["precision"],
                    "warm_start": spec["warm"], "learned_start": spec["learned"], "block_cg": spec["block"],
+                   "island_correction": spec["islands"],
                    "resolution": R,
                    "levels": NLEVELS, "rtol": 1e-6},
         "ch_rel_err_vs_fp64_tight_solve": g["ch_rel_err"],

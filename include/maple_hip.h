@@ -86,6 +86,22 @@ int maple_set_solver(maple_ctx* ctx, double omega, int nsmooth, int warm_start);
 #define MAPLE_SMOOTH_L1_JACOBI 1
 int maple_set_smoother(maple_ctx* ctx, int kind);
 
+/* Island correction of the preconditioner (off by default, the reference's
+ * V-cycle alone).  On: every homogenization finds the small solid islands of
+ * the density field -- connected clusters (26-neighbour) of at most 16 elements
+ * with rho > threshold and at most 32 nodes, touching the rest of the cell only
+ * through elements at or below the threshold -- inverts each island's block of
+ * the level-0 operator (its nodes, every element touching them) in FP64, and
+ * adds z += A_i^{-1} r|_i to every preconditioner application.  The
+ * preconditioner stays SPD, so CG converges to the same tolerance; islands held
+ * by 1e-9 voids otherwise cost CG iterations for each of their soft modes.  Up
+ * to 256 islands per cell are corrected; the first call with on = 1 allocates
+ * the island buffers.  No counterpart in the reference, whose V-cycle
+ * (EBE_GMG.py:886-984) has no such term. */
+int maple_set_islands(maple_ctx* ctx, int on, double threshold);
+/* Islands corrected in the last homogenization (all cells). */
+int maple_num_islands(const maple_ctx* ctx);
+
 /* CG variant: 0 = Fletcher-Reeves beta, 1 = flexible (Polak-Ribiere) beta, which
  * stays valid for a preconditioner that is not exactly symmetric (one extra dot
  * product per iteration), -1 = flexible in MAPLE_PREC_MIXED only (default),
@@ -129,7 +145,13 @@ int maple_set_coarse_refresh(maple_ctx* ctx, int every);
  * which gather their shared neighbours once), 6 or 7 (2x2 nodes in x and z per
  * thread, three or two load cases).  Default: 5.  The thermal operator always
  * uses the element-row kernel (variant 0, or 1 when 1 is set). */
+/* MAPLE_KERNEL_COARSE_OPERATOR: the elastic Galerkin stencil product of the
+ * coarse levels (the coarse matvec of the V-cycle, EBE_GMG.py:886-984), variant
+ * 0 (one thread per node row, three times the waves), 1 (one thread per node,
+ * each neighbour gathered once for the three rows) or 2 (1 for launches of 1024
+ * or more blocks, 0 below; the default). */
 #define MAPLE_KERNEL_FINE_OPERATOR 0
+#define MAPLE_KERNEL_COARSE_OPERATOR 1
 int maple_set_variant(maple_ctx* ctx, int kernel, int variant);
 
 /* Homogenize: for every cell, solve the 6 elastic unit-strain (and/or 3 thermal

@@ -25,6 +25,7 @@ MAPLE_PREC_MIXED = 1
 MAPLE_SMOOTH_JACOBI = 0
 MAPLE_SMOOTH_L1_JACOBI = 1
 MAPLE_KERNEL_FINE_OPERATOR = 0
+MAPLE_KERNEL_COARSE_OPERATOR = 1
 
 _P = C.c_void_p
 _D = C.c_double
@@ -43,6 +44,8 @@ SIGNATURES = {
     "maple_set_element_matrices": (_I, [_P, _I, _DP, _DP, _DP, _DP, _DP, _DP]),
     "maple_set_solver": (_I, [_P, _D, _I, _I]),
     "maple_set_smoother": (_I, [_P, _I]),
+    "maple_set_islands": (_I, [_P, _I, _D]),
+    "maple_num_islands": (_I, [_P]),
     "maple_set_cg": (_I, [_P, _I]),
     "maple_set_mixed_fallback": (_I, [_P, _I]),
     "maple_set_concurrency": (_I, [_P, _I]),

@@ -43,7 +43,7 @@ def max_levels(r: int, min_coarse: int = 4) -> int:
 class CellBatch:
     def __init__(self, dims: Sequence, nlevels: int = 4, device=None, precision: str = "fp64",
                  omega: Optional[float] = None, nsmooth: int = 2, warm_start: bool = False,
-                 smoother: str = "l1-jacobi", block_cg: bool = False):
+                 smoother: str = "l1-jacobi", block_cg: bool = False, islands: bool = False):
         self.lib = L.load()
         dl = []
         for d in dims:
@@ -76,6 +76,8 @@ class CellBatch:
         self.set_solver(omega, nsmooth, warm_start)
         if block_cg:
             self.set_cg(block=True)
+        if islands:
+            self.set_islands(True)
 
     # -- lifetime ---------------------------------------------------------
     def close(self):
@@ -124,6 +126,15 @@ class CellBatch:
     def set_concurrency(self, on: bool = True):
         """Solve elastic and thermal at the same time (second stream and host thread) or in turn."""
         L.check(self.lib.maple_set_concurrency(self._h, int(bool(on))))
+
+    def set_islands(self, on: bool = True, threshold: float = 1e-4):
+        """Island correction of the preconditioner: small solid islands (rho > threshold,
+        at most 16 elements and 32 nodes) are solved exactly beside the V-cycle."""
+        L.check(self.lib.maple_set_islands(self._h, int(bool(on)), float(threshold)))
+
+    def num_islands(self) -> int:
+        """Islands corrected in the last homogenization."""
+        return int(self.lib.maple_num_islands(self._h))
 
     def set_mixed_fallback(self, iters: int = 300):
         """Mixed form: finish a solve on the FP64 V-cycle once it has run `iters` CG

@@ -104,6 +104,8 @@ struct Phys {
   size_t stage_bytes = 0;
   int stage_slots = 0, stage_next = 0, nlive = 0;
   bool have_start = false;
+  double* isl_inv = nullptr;  // island correction: inverse blocks of this physics, per island
+  int* isl_valid = nullptr;
   long long setups = 0;    // hierarchy builds so far (drives the coarsest-inverse refresh)
 };
 
@@ -134,6 +136,15 @@ struct maple_ctx {
   hipStream_t side = nullptr;                  // the thermal solve's stream when both physics run
   hipEvent_t fork = nullptr, join = nullptr;
   bool concurrent = true;                      // solve elastic and thermal at the same time
+  // island correction of the preconditioner (islands.hip)
+  bool islands = false;
+  double isl_thr = 1e-4;
+  int isl_max = 0;                              // island slots (0: not allocated)
+  int *isl_lab = nullptr, *isl_lab2 = nullptr, *isl_cnt = nullptr, *isl_bad = nullptr, *isl_slot = nullptr;
+  int* isl_count_d = nullptr;
+  int* isl_count_h = nullptr;                   // pinned
+  Island* isl = nullptr;
+  int nisl = 0;                                 // islands of the current density
   bool timing = false;
   Timer timers[3];   // 0: FP64 elastic level-0 operator, 1: FP64 thermal, 2: FP32 elastic
   unsigned long long* work_d = nullptr;   // per timer: elements x load cases of the timed launches
@@ -465,12 +476,20 @@ T* vcycle_t(maple_ctx* c, Phys& P, VC<T>& V, int l, const T* b, hipStream_t s) {
 
 // Preconditioner on the CG residual r (FP64 in, FP64 out): the FP64 V-cycle, or the
 // FP32 one between conversions.
+// With the island correction on, z += A_i^{-1} r on every small island (FP64).
 double* precondition(maple_ctx* c, Phys& P, const double* r, bool use64, hipStream_t s) {
-  if (use64) return vcycle_t<double>(c, P, P.v64, 0, r, s);
-  vec_copy_cases<float, double>(P.ln[0], P.nd, P.nc, r, P.v32.B[0], s);
-  float* z = vcycle_t<float>(c, P, P.v32, 0, P.v32.B[0], s);
-  vec_copy_cases<double, float>(P.ln[0], P.nd, P.nc, z, P.z64, s);
-  return P.z64;
+  double* z;
+  if (use64) {
+    z = vcycle_t<double>(c, P, P.v64, 0, r, s);
+  } else {
+    vec_copy_cases<float, double>(P.ln[0], P.nd, P.nc, r, P.v32.B[0], s);
+    float* z32 = vcycle_t<float>(c, P, P.v32, 0, P.v32.B[0], s);
+    vec_copy_cases<double, float>(P.ln[0], P.nd, P.nc, z32, P.z64, s);
+    z = P.z64;
+  }
+  if (c->islands && c->nisl > 0)
+    island_apply(P.nd, P.nc, c->isl, P.isl_valid, c->nisl, P.isl_inv, c->lv[0].Ntot, P.done_d, P.nc, r, z, s);
+  return z;
 }
 
 // Launch tables over every cell (start of a solve).
@@ -710,6 +729,8 @@ int solve_physics(maple_ctx* c, int p, const double* rho, double rtol, int maxit
                   hipStream_t s) {
   Phys& P = c->ph[p];
   setup_hierarchy(c, P, rho, s);
+  if (c->islands && c->nisl > 0)
+    island_setup(P.nd, c->lv[0].d, c->isl, c->nisl, P.Ke, rho, P.isl_inv, P.isl_valid, s);
   HIPCHK(hipGetLastError());
   int status = solve_cases(c, P, rho, rtol, maxit, s);
   if (status == MAPLE_ENOCONV) {
@@ -725,6 +746,22 @@ int solve_physics(maple_ctx* c, int p, const double* rho, double rtol, int maxit
     for (int i = 0; i < c->ncells; ++i)
       for (int j = 0; j < P.nc; ++j) iters_host[i * 9 + (p == 0 ? 0 : 6) + j] = P.hsc[i * P.nc + j].iters;
   return status;
+}
+
+// Islands of this density field (both physics use the same ones).  One host
+// synchronisation: the island count sizes the setup and correction launches.
+int find_islands(maple_ctx* c, const double* rho, hipStream_t s) {
+  c->nisl = 0;
+  if (!c->islands) return 0;
+  vec_zero<int>(1, c->isl_count_d, s);
+  island_detect(c->lv[0].node, rho, c->isl_thr, c->isl_lab, c->isl_lab2, c->isl_cnt, c->isl_bad, c->isl_slot,
+                c->isl, c->isl_count_d, c->isl_max, s);
+  HIPCHK(hipMemcpyAsync(c->isl_count_h, c->isl_count_d, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  c->nisl = std::min(*c->isl_count_h, c->isl_max);
+  island_nodes(c->lv[0].d, c->isl, c->nisl, s);
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 int check_ctx(const maple_ctx* c) {
@@ -819,6 +856,8 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
       if ((rc = make_launch(c, L, 512, false, &L.dxblk, &L.pair, &L.cnt_pair))) return bail(rc);
       if ((rc = make_launch(c, L, 1024, false, &L.dqblk, &L.quad, &L.cnt_quad))) return bail(rc);
       L.fine.var = 5;   // default level-0 elastic operator: 2x1 node blocks, three load cases per thread
+    } else {
+      L.node.var = 2;   // default coarse elastic operator: node or row threads by launch size
     }
   }
   // partial-sum scratch: 80 doubles per block of the largest launch (block CG: two 6x6 Grams)
@@ -913,11 +952,21 @@ int maple_set_solver(maple_ctx* c, double omega, int nsmooth, int warm_start) {
 
 int maple_set_variant(maple_ctx* c, int kernel, int variant) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
-  if (kernel != MAPLE_KERNEL_FINE_OPERATOR) return fail(MAPLE_EINVAL, "unknown kernel id");
-  if (variant < 0 || variant > 7) return fail(MAPLE_EINVAL, "unknown variant");
-  c->lv[0].fine.var = variant;
-  for (Phys& P : c->ph) P.lf.var = variant;
-  return 0;
+  if (kernel == MAPLE_KERNEL_FINE_OPERATOR) {
+    if (variant < 0 || variant > 7) return fail(MAPLE_EINVAL, "unknown variant");
+    c->lv[0].fine.var = variant;
+    for (Phys& P : c->ph) P.lf.var = variant;
+    return 0;
+  }
+  if (kernel == MAPLE_KERNEL_COARSE_OPERATOR) {
+    if (variant < 0 || variant > 2) return fail(MAPLE_EINVAL, "unknown variant");
+    for (int l = 1; l < c->nlevels; ++l) {
+      c->lv[l].node.var = variant;
+      for (Phys& P : c->ph) P.ln[l].var = variant;
+    }
+    return 0;
+  }
+  return fail(MAPLE_EINVAL, "unknown kernel id");
 }
 
 int maple_set_coarse_refresh(maple_ctx* c, int every) {
@@ -948,6 +997,43 @@ int maple_set_cg(maple_ctx* c, int flexible) {
   return 0;
 }
 
+int maple_set_islands(maple_ctx* c, int on, double threshold) {
+  TRY(check_ctx(c));
+  if (!(threshold > 0.0)) return fail(MAPLE_EINVAL, "island threshold must be positive");
+  c->isl_thr = threshold;
+  c->islands = false;
+  if (!on) return 0;
+  if (c->isl_max > 0) {
+    c->islands = true;
+    return 0;
+  }
+  TRY(set_device(c));
+  const long long E = c->lv[0].Ntot;
+  const int maxi = 256 * c->ncells;
+  TRY(dalloc(c, &c->isl_lab, (size_t)E));
+  TRY(dalloc(c, &c->isl_lab2, (size_t)E));
+  TRY(dalloc(c, &c->isl_cnt, (size_t)E));
+  TRY(dalloc(c, &c->isl_bad, (size_t)E));
+  TRY(dalloc(c, &c->isl_slot, (size_t)E));
+  TRY(dalloc(c, &c->isl_count_d, 1));
+  TRY(dalloc(c, &c->isl, (size_t)maxi));
+  for (Phys& P : c->ph) {
+    const size_t nm = (size_t)P.nd * MAPLE_ISL_MAXN;
+    TRY(dalloc(c, &P.isl_inv, (size_t)maxi * nm * nm));
+    TRY(dalloc(c, &P.isl_valid, (size_t)maxi));
+  }
+  void* hp = nullptr;
+  if (hipHostMalloc(&hp, sizeof(int), hipHostMallocDefault) != hipSuccess)
+    return fail(MAPLE_ENOMEM, "pinned host allocation failed");
+  c->host_allocs.push_back(hp);
+  c->isl_count_h = (int*)hp;
+  c->isl_max = maxi;
+  c->islands = true;
+  return 0;
+}
+
+int maple_num_islands(const maple_ctx* c) { return c ? c->nisl : 0; }
+
 int maple_set_smoother(maple_ctx* c, int kind) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
   if (kind != MAPLE_SMOOTH_JACOBI && kind != MAPLE_SMOOTH_L1_JACOBI) return fail(MAPLE_EINVAL, "unknown smoother");
@@ -962,6 +1048,7 @@ int maple_homogenize(maple_ctx* c, const double* rho, int physics, double rtol, 
   if (!(physics & (MAPLE_ELASTIC | MAPLE_THERMAL))) return fail(MAPLE_EINVAL, "no physics selected");
   if (!(rtol > 0.0) || maxit < 1) return fail(MAPLE_EINVAL, "need rtol > 0 and maxit >= 1");
   hipStream_t s = (hipStream_t)stream;
+  TRY(find_islands(c, rho, s));
   const bool both = (physics & MAPLE_ELASTIC) && (physics & MAPLE_THERMAL);
   if (!both || !c->concurrent) {
     int status = 0;

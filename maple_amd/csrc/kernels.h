@@ -201,3 +201,25 @@ void oc_pass(const Launch& L, const OCState* st, const double* x, const double* 
 void oc_walk(const int* blk_start, const LevelCell* cells, int ncells, int depth, const double* partial,
              OCState* st, hipStream_t s);
 void oc_final(const Launch& L, const OCState* st, const double* x, const double* a, double* xnew, hipStream_t s);
+
+// ---- island correction of the preconditioner (islands.hip) -------------------
+#define MAPLE_ISL_MAXE 16     // elements of an island
+#define MAPLE_ISL_MAXN 32     // nodes of an island (its block is at most 96 x 96)
+#define MAPLE_ISL_ROUNDS 8    // label propagation rounds (even)
+struct Island {
+  int cell;                      // cell id
+  int ne, nn;                    // elements; nodes (0: not corrected)
+  int pad_;
+  int elems[MAPLE_ISL_MAXE];     // global element ids, sorted
+  int nodes[MAPLE_ISL_MAXN];     // global node ids, sorted
+};
+// L0: the full level-0 node launch; lab, lab2, cnt, bad, slot: Etot ints each;
+// nisl (device) counts the islands found (slots beyond maxi are dropped)
+void island_detect(const Launch& L0, const double* rho, double thr, int* lab, int* lab2, int* cnt, int* bad,
+                   int* slot, Island* isl, int* nisl, int maxi, hipStream_t s);
+void island_nodes(const LevelCell* cells, Island* isl, int n, hipStream_t s);
+// inv: n x (nd*MAPLE_ISL_MAXN)^2 doubles; valid: n ints
+void island_setup(int nd, const LevelCell* cells, Island* isl, int n, const double* Ke, const double* rho,
+                  double* inv, int* valid, hipStream_t s);
+void island_apply(int nd, int ncase, const Island* isl, const int* valid, int n, const double* inv, long long Ntot,
+                  const int* done, int ndc, const double* r, double* z, hipStream_t s);

@@ -59,7 +59,7 @@ class _Homogenize(torch.autograd.Function):
 class IHD(nn.Module):
     def __init__(self, cfg, device, *, rtol: float = 1e-6, maxit: int = 1000, nlevels: Optional[int] = None,
                  precision: str = "fp64", warm_start: bool = False, omega: Optional[float] = None, nsmooth: int = 2,
-                 E0: float = 1.0, nu: float = 0.3, kappa0: float = 1.0):
+                 E0: float = 1.0, nu: float = 0.3, kappa0: float = 1.0, islands: bool = False):
         super().__init__()
         self.cfg = cfg
         self.r = self.resolution = int(cfg.resolution)
@@ -83,7 +83,7 @@ class IHD(nn.Module):
         self.rtol = float(rtol)
         self.maxit = int(maxit)
         self.cells = CellBatch([self.r], nlevels=self.nlevels, device=self.device, precision=precision,
-                               omega=omega, nsmooth=nsmooth, warm_start=warm_start)
+                               omega=omega, nsmooth=nsmooth, warm_start=warm_start, islands=islands)
         if (E0, nu, kappa0) != (1.0, 0.3, 1.0):
             self.cells.set_element_matrices(0, Ke24, Fe24, X024, Ke8, Fe8, X08)
         self.last_iters = None

@@ -92,6 +92,31 @@ def test_homogenize_node_variants(variant, precision):
     cb.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_coarse_operator_variant(precision):
+    """The node-per-thread coarse stencil product (all rows of a node in one thread)
+    inside the whole solve against the oracle, and against the row-per-thread form."""
+    import maple_amd
+    r, nlev = 16, 4
+    print(f"[coarse variant] {precision}", flush=True)
+    rho = _rho(r, 12)
+    out = []
+    for v in (0, 1):
+        cb = maple_amd.CellBatch([r], nlevels=nlev, device=DEV, precision=precision)
+        cb.set_variant(maple_amd._lib.MAPLE_KERNEL_COARSE_OPERATOR, v)
+        CH, kap, it = cb.homogenize(_t(rho), rtol=1e-10)
+        out.append((CH[0].cpu().numpy(), kap[0].cpu().numpy(), it))
+        cb.close()
+    cs = ohomog.CellSolver((r, r, r), nlevels=3, rtol=1e-12)
+    CHo, _, _ = cs.elastic(rho)
+    ko, _, _ = cs.thermal(rho)
+    for CH, kap, _ in out:
+        assert np.linalg.norm(CH - CHo) <= 1e-9 * np.linalg.norm(CHo)
+        assert np.linalg.norm(kap - ko) <= 1e-9 * np.linalg.norm(ko)
+    # the same operator up to rounding: the same CG iteration counts
+    assert np.abs(out[0][2].astype(int) - out[1][2].astype(int)).max() <= 1
+
+
 @pytest.mark.parametrize("r,nlev,seed", [(8, 2, 0), (8, 3, 1), (16, 3, 2), (16, 4, 3)])
 def test_homogenize_parity(r, nlev, seed):
     print(f"[homogenize] r={r} nlev={nlev}", flush=True)
