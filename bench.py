@@ -11,7 +11,8 @@ disconnected islands slow the cell solves down.
 
 Forms (one JSON line; ``value`` is the headline form):
 * ``plain``    one cell, FP64, zero start every solve: the reference's method on the GPU.
-* ``headline`` 16 cells advanced together (one set of launches per OC step),
+* ``headline`` 16 cells in 4 concurrent groups of 4 (each group one set of
+               launches per OC step, its own stream and host thread),
                FP64 CG around an FP32 multigrid preconditioner with the FP64
                island correction (small solid islands solved exactly beside the
                V-cycle), each solve starting from the best of zero, the previous
@@ -21,8 +22,8 @@ Forms (one JSON line; ``value`` is the headline form):
   ``fp64-islands``, ``fp64-block``, ``mixed-warm``, ``mixed-warm-block``,
   ``mixed-warm-learned``, ``batched-mixed-warm``, ``batched-mixed-warm-islands``,
   ``batched-mixed-warm-block``, ``headline-block`` with block CG over the load
-  cases); --config4 adds the 64-cell mixed-resolution batch (64^3-equivalent
-  iterations / s).
+  cases, ``headline-lockstep`` with all 16 cells in one group); --config4 adds
+  the 64-cell mixed-resolution batch (64^3-equivalent iterations / s).
 
 Accuracy: the last iterate's C_H and kappa of the headline form (every cell)
 against a tight (rtol 1e-11) FP64 solve of the same densities.  ``roofline`` is
@@ -50,8 +51,8 @@ PEAK = {"fp64": 78.6, "mixed": 157.3}
 FLOP_PER_ELEM_ELASTIC = 1152      # 24 rows x 24 cols x 2 per element and load case
 # HBM bytes per 64^3 cell and launch of the level-0 elastic operator, from rocprofv3
 # FETCH_SIZE + WRITE_SIZE runs calibrated against a copy kernel
-# (profiles/fine_operator_64cube_hbm_counters.md): FP64 product 87.6 MB, FP32 Jacobi sweep 57.1 MB
-TRAFFIC_PER_CELL = {"fp64": 87.6e6, "mixed": 57.1e6}
+# (profiles/fine_operator_64cube_hbm_counters.md): FP64 product 105.5 MB, FP32 Jacobi sweep 81.2 MB
+TRAFFIC_PER_CELL = {"fp64": 105.5e6, "mixed": 81.2e6}
 
 
 def _x0(ncells):
@@ -62,58 +63,72 @@ def _x0(ncells):
 
 
 def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", warm: bool = False,
-             learned: bool = False, block: bool = False, islands: bool = False, maxit: int = 5000,
+             learned: bool = False, block: bool = False, islands: bool = False, groups: int = 1, maxit: int = 5000,
              profile_kernel: bool = True):
+    """groups > 1: the cells advance in that many concurrent groups (maple_amd.concurrent),
+    each its own batch, stream and host thread; every cell's trajectory is unchanged."""
     import numpy as np
     import torch
 
     import maple_amd
+    from maple_amd.concurrent import ConcurrentOC
 
     dev = torch.device("cuda", 0)
+    per = ncells // groups
 
     def make():
-        cells = maple_amd.CellBatch([R] * ncells, nlevels=NLEVELS, device=dev, warm_start=warm, precision=precision,
-                                    block_cg=block)
-        if islands:
-            cells.set_islands(True)
-        start = None
-        if learned:
-            from maple_amd import learned as ml
-            start = ml.LearnedStart(cells, ml.UNet.load(max_batch=min(16, ncells), device=dev))
-        drv = maple_amd.OCDriver(cells, OBJ, 0.25, move=0.2, xmin=1e-3, filter_k=3, rtol=1e-6, maxit=maxit,
-                                 start=start)
-        return cells, drv
+        out = []
+        for g in range(groups):
+            cells = maple_amd.CellBatch([R] * per, nlevels=NLEVELS, device=dev, warm_start=warm,
+                                        precision=precision, block_cg=block, islands=islands)
+            start = None
+            if learned:
+                from maple_amd import learned as ml
+                start = ml.LearnedStart(cells, ml.UNet.load(max_batch=min(16, per), device=dev))
+            drv = maple_amd.OCDriver(cells, OBJ, 0.25, move=0.2, xmin=1e-3, filter_k=3, rtol=1e-6, maxit=maxit,
+                                     start=start)
+            out.append((cells, drv))
+        return out
 
     x0 = torch.from_numpy(_x0(ncells)).to(dev)
+    xs = [x0[g * per * R ** 3:(g + 1) * per * R ** 3].clone() for g in range(groups)]
     if warmup:
-        cells, drv = make()
-        x = x0.clone()
-        for _ in range(warmup):
-            x, _ = drv.step(x)
+        sets = make()
+        ConcurrentOC([(c, d, x.clone()) for (c, d), x in zip(sets, xs)]).run(warmup)
         torch.cuda.synchronize()
-        cells.close()
-    cells, drv = make()
+        for c, _ in sets:
+            c.close()
+    sets = make()
     kind = 0 if precision == "fp64" else 2
     if profile_kernel:
-        cells.timing(True)
-        cells.timing_read(kind)
-    iters = []
-    x = x0.clone()
+        for c, _ in sets:
+            c.timing(True)
+            c.timing_read(kind)
+    iters = np.zeros((groups, steps, per, 9), dtype=np.int64)
+
+    def on_step(g, k, info):
+        iters[g, k] = info["iters"]
+
+    co = ConcurrentOC([(c, d, x.clone()) for (c, d), x in zip(sets, xs)])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        x, info = drv.step(x)
-        iters.append(info["iters"].max(axis=0).tolist())
+    res = co.run(steps, on_step)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    count, kms, work = cells.timing_read(kind, work=True) if profile_kernel else (0, 0.0, 0)
-    cells.timing(False)
+    count, kms, work = 0, 0.0, 0
+    if profile_kernel:
+        for c, _ in sets:
+            n_, ms_, w_ = c.timing_read(kind, work=True)
+            count, kms, work = count + n_, kms + ms_, work + w_
+            c.timing(False)
     # accuracy: the last step's C_H and kappa (solved at info["rho"]) against a tight FP64 solve
-    rho = info["rho"]
+    rho = torch.cat([r[1]["rho"] for r in res])
+    CHl = np.concatenate([r[1]["CH"] for r in res])
+    kl = np.concatenate([r[1]["kappa"] for r in res])
     ref = maple_amd.CellBatch([R] * ncells, nlevels=NLEVELS, device=dev)
     CHt, kt, _ = ref.homogenize(rho, ("elastic", "thermal"), rtol=1e-11, maxit=4 * maxit, strict=False)
-    CH6 = torch.from_numpy(info["CH"]).to(dev)
-    k6 = torch.from_numpy(info["kappa"]).to(dev)
+    CH6 = torch.from_numpy(CHl).to(dev)
+    k6 = torch.from_numpy(kl).to(dev)
     err = 0.0
     for c in range(ncells):
         err = max(err, float(torch.linalg.norm(CH6[c] - CHt[c]) / torch.linalg.norm(CHt[c])),
@@ -121,13 +136,18 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
     ref.close()
     src = None
     if learned or warm:
-        src = {p: np.bincount(cells.start_sources(p).ravel(), minlength=3).tolist() for p in ("elastic", "thermal")}
-    cells.close()
+        src = {p: np.sum([np.bincount(c.start_sources(p).ravel(), minlength=3) for c, _ in sets], axis=0).tolist()
+               for p in ("elastic", "thermal")}
+    J_last = float(res[0][1]["J"][0])
+    for c, _ in sets:
+        c.close()
     dt = t1 - t0
-    it = np.array(iters)
-    out = {"value": ncells * steps / dt, "ms_per_step": dt * 1e3 / steps, "cells": ncells,
-           "iters_el_max": int(it[:, :6].max()), "iters_el_mean": float(it[:, :6].max(axis=1).mean()),
-           "iters_th_mean": float(it[:, 6:].max(axis=1).mean()), "ch_rel_err": err, "J_last": float(info["J"][0]),
+    # per step: the largest count over the cells of a group, averaged over the groups
+    it_el = iters[..., :6].max(axis=(2, 3))
+    it_th = iters[..., 6:].max(axis=(2, 3))
+    out = {"value": ncells * steps / dt, "ms_per_step": dt * 1e3 / steps, "cells": ncells, "groups": groups,
+           "iters_el_max": int(it_el.max()), "iters_el_mean": float(it_el.mean()),
+           "iters_th_mean": float(it_th.mean()), "ch_rel_err": err, "J_last": J_last,
            "last_starts": src}
     if count:
         # launches skip converged cells and load cases: the work of each launch is
@@ -135,8 +155,8 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
         avg_s = kms / count * 1e-3
         flops = work / count * FLOP_PER_ELEM_ELASTIC
         tname = "double" if precision == "fp64" else "float"
-        out["kernel"] = {"name": f"k_fine_apply<3,{tname},*> (level-0 elastic operator, up to 6 load cases x {ncells} "
-                                 f"cells per launch)",
+        out["kernel"] = {"name": f"k_fine_nodeb<{tname},*,3,2,1> (level-0 elastic operator, node blocks, "
+                                 f"up to 6 load cases x {per} cells per launch)",
                          "launches": count, "avg_us": avg_s * 1e6, "flop_per_launch": flops,
                          "elem_cases_per_launch": work / count,
                          "achieved_tflops": flops / avg_s / 1e12, "peak": PEAK[precision],
@@ -189,8 +209,10 @@ def cpu_baseline(n_iter: int):
 
 FORMS = {
     "plain": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False, islands=False),
-    "headline": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True),
-    "headline-block": dict(ncells=16, precision="mixed", warm=True, learned=True, block=True, islands=True),
+    "headline": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True, groups=4),
+    "headline-lockstep": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True),
+    "headline-block": dict(ncells=16, precision="mixed", warm=True, learned=True, block=True, islands=True,
+                           groups=4),
     "fp64-warm": dict(ncells=1, precision="fp64", warm=True, learned=False, block=False, islands=False),
     "fp64-islands": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False, islands=True),
     "fp64-block": dict(ncells=1, precision="fp64", warm=False, learned=False, block=True, islands=False),
@@ -210,7 +232,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--forms", default="plain,headline", help="comma list of " + ", ".join(FORMS))
+    ap.add_argument("--forms", default="plain,headline,headline-lockstep", help="comma list of " + ", ".join(FORMS))
     ap.add_argument("--config4", action="store_true", help="also run the 64-cell mixed-resolution batch")
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -250,23 +272,28 @@ def main():
                                "filter k=3, 4-level MG-PCG rtol 1e-6",
                    "form": head, "cells": spec["ncells"], "precision": spec["precision"],
                    "warm_start": spec["warm"], "learned_start": spec["learned"], "block_cg": spec["block"],
-                   "island_correction": spec["islands"],
+                   "island_correction": spec["islands"], "groups": spec.get("groups", 1),
                    "resolution": R,
                    "levels": NLEVELS, "rtol": 1e-6},
         "ch_rel_err_vs_fp64_tight_solve": g["ch_rel_err"],
-        "forms": {f: {k: res[f][k] for k in ("value", "ms_per_step", "cells", "ch_rel_err", "iters_el_mean",
+        "forms": {f: {k: res[f][k] for k in ("value", "ms_per_step", "cells", "groups", "ch_rel_err", "iters_el_mean",
                                              "iters_th_mean", "iters_el_max", "last_starts")}
                   | {"kernel_avg_us": res[f].get("kernel", {}).get("avg_us")} for f in forms},
     }
     if "plain" in res and head != "plain":
         line["speedup_vs_plain_fp64"] = g["value"] / res["plain"]["value"]
-    k = g.get("kernel")
+    # roofline of the dominant kernel: from the lockstep run of the same workload when it
+    # was run (its launches do not overlap another group's, so a launch's HIP-event span
+    # is its own duration), else from the headline run itself
+    rl_form = "headline-lockstep" if (head == "headline" and "headline-lockstep" in res) else head
+    k = res[rl_form].get("kernel")
     if k:
         line["roofline"] = {"bound": "mfma", "achieved": k["achieved_tflops"], "peak": k["peak"],
                             "unit": "TFLOP/s", "frac": k["achieved_tflops"] / k["peak"], "traffic": k["traffic"],
                             "traffic_unit": "bytes per launch (rocprofv3 PMC, copy-calibrated)",
                             "kernel": k["name"], "avg_us": k["avg_us"], "launches": k["launches"],
                             "algorithmic_flop_per_launch": k["flop_per_launch"],
+                            "measured_on_form": rl_form,
                             "roof": "vector FMA and MFMA share the MI355X peak for this precision "
                                     "(FP64 78.6, FP32 157.3 TFLOP/s)"}
     if args.config4:
