@@ -492,6 +492,34 @@ double* precondition(maple_ctx* c, Phys& P, const double* r, bool use64, hipStre
   return z;
 }
 
+// CG's preconditioned residual: FP64 (FP64 V-cycle) or, in the mixed form, the FP32
+// V-cycle's own output, never widened (the CG reads it as FP32).
+struct ZVec {
+  double* d = nullptr;
+  float* f = nullptr;
+};
+
+// r32_ready: the FP32 copy of r is already in the V-cycle's input (the mixed update wrote it)
+ZVec precondition_cg(maple_ctx* c, Phys& P, const double* r, bool use64, bool r32_ready, hipStream_t s) {
+  ZVec z;
+  if (use64) {
+    z.d = vcycle_t<double>(c, P, P.v64, 0, r, s);
+    if (c->islands && c->nisl > 0)
+      island_apply(P.nd, P.nc, c->isl, P.isl_valid, c->nisl, P.isl_inv, c->lv[0].Ntot, P.done_d, P.nc, r, z.d, s);
+    return z;
+  }
+  if (!r32_ready) vec_copy_cases<float, double>(P.ln[0], P.nd, P.nc, r, P.v32.B[0], s);
+  z.f = vcycle_t<float>(c, P, P.v32, 0, P.v32.B[0], s);
+  if (c->islands && c->nisl > 0)
+    island_apply(P.nd, P.nc, c->isl, P.isl_valid, c->nisl, P.isl_inv, c->lv[0].Ntot, P.done_d, P.nc, r, z.f, s);
+  return z;
+}
+
+void dot_rz(Phys& P, const double* r, const ZVec& z, double* partial, hipStream_t s) {
+  if (z.f) vec_dot<double, float>(P.ln[0], P.nd, P.nc, r, z.f, partial, s);
+  else vec_dot<double, double>(P.ln[0], P.nd, P.nc, r, z.d, partial, s);
+}
+
 // Launch tables over every cell (start of a solve).
 void full_launches(maple_ctx* c, Phys& P) {
   for (size_t l = 0; l < P.ln.size(); ++l) {
@@ -574,6 +602,8 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   const bool flex = c->flexible < 0 ? c->prec != MAPLE_PREC_FP64 : c->flexible != 0;
   bool use64 = c->prec == MAPLE_PREC_FP64;
   const long long n0 = (long long)P.nc * P.nd * L0.Ntot;
+  // second partial-sum area of the solve (the mixed update's fused flexible dot)
+  double* partial2 = P.partial + (size_t)c->lv[0].node.nblocks * MAPLE_MAX_CASES;
   vec_zero<int>((long long)c->ncells * P.nc, P.done_d, s);   // no case is converged before the start
   full_launches(c, P);
   fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
@@ -596,11 +626,12 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   P.have_start = false;
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, P.partial, s);
   reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RR, s);
-  double* z = precondition(c, P, P.r, use64, s);
-  vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, z, P.partial, s);
+  ZVec z = precondition_cg(c, P, P.r, use64, false, s);
+  dot_rz(P, P.r, z, P.partial, s);
   reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RZ_NEW, s);
   pcg_init_scalars(c->ncells, P.nc, rtol, P.atol2, P.sc, P.done_d, s);
-  vec_copy<double, double>(n0, z, P.p, s);
+  if (z.f) vec_copy<double, float>(n0, z.f, P.p, s);
+  else vec_copy<double, double>(n0, z.d, P.p, s);
   TRY(enqueue_poll(c, P, 0, s));
   bool all = false;
   TRY(poll_done(c, P, 0, &all));
@@ -608,14 +639,24 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
     apply_fine64(c, P, 1, rho, P.p, nullptr, P.q, P.partial, s);
     reduce_partials(op_launch(c, P, false), P.nc, P.partial, P.sc, SC_PQ, s);
     pcg_alpha(c->ncells, P.nc, P.sc, s);
-    pcg_update_ur(P.ln[0], P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, P.partial, s);
+    // mixed form: the update also writes the FP32 preconditioner's input and, from
+    // the previous FP32 z, the flexible dot (r_new, z_old)
+    bool r32_ready = false, rzo_done = false;
+    if (!use64 && pcg_update_ur_mixed(P.ln[0], P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, P.v32.B[0],
+                                      flex ? z.f : nullptr, P.partial, partial2, s) == 0) {
+      r32_ready = true;
+      rzo_done = flex && z.f != nullptr;
+    } else {
+      pcg_update_ur(P.ln[0], P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, P.partial, s);
+    }
     reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RR, s);
+    if (rzo_done) reduce_partials(P.ln[0], P.nc, partial2, P.sc, SC_RZO, s);
     pcg_check(c->ncells, P.nc, P.sc, P.done_d, s);
     const int slot = (it + 1) & 1;
     TRY(enqueue_poll(c, P, slot, s));
     if (it + 1 < maxit) {
-      if (flex) {
-        vec_dot<double, double>(P.ln[0], P.nd, P.nc, P.r, z, P.partial, s);
+      if (flex && !rzo_done) {
+        dot_rz(P, P.r, z, P.partial, s);
         reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RZO, s);
       }
       // An FP32 V-cycle cannot resolve the soft modes of solid islands held by 1e-9
@@ -625,11 +666,18 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
         setup_fp64(c, P, rho, s, true);
         use64 = true;
       }
-      z = precondition(c, P, P.r, use64, s);
-      vec_dot<double, double>(P.ln[0], P.nd, P.nc, P.r, z, P.partial, s);
+      z = precondition_cg(c, P, P.r, use64, r32_ready && !use64, s);
+      dot_rz(P, P.r, z, P.partial, s);
       reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RZ_NEW, s);
       pcg_beta(c->ncells, P.nc, flex, P.sc, s);
-      pcg_update_p(P.ln[0], P.nd, P.nc, P.sc, z, P.p, s);
+      if (!(z.f && pcg_update_p32(P.ln[0], P.nd, P.nc, P.sc, z.f, P.p, s) == 0)) {
+        if (z.f) {   // shapes without a fused kernel: widen first
+          vec_copy_cases<double, float>(P.ln[0], P.nd, P.nc, z.f, P.z64, s);
+          pcg_update_p(P.ln[0], P.nd, P.nc, P.sc, P.z64, P.p, s);
+        } else {
+          pcg_update_p(P.ln[0], P.nd, P.nc, P.sc, z.d, P.p, s);
+        }
+      }
     }
     // the device is already working on the next iteration while we look at this one
     TRY(poll_done(c, P, slot, &all));

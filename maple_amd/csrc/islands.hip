@@ -255,11 +255,11 @@ __global__ __launch_bounds__(256) void k_isl_setup(const LevelCell* __restrict__
 }
 
 // z += A_i^{-1} r on island i for load case blockIdx.y (skipped once converged)
-template <int ND>
+template <int ND, typename Tz>
 __global__ __launch_bounds__(128) void k_isl_apply(const Island* __restrict__ isl, const int* __restrict__ valid,
                                                    const double* __restrict__ inv, long long Ntot,
                                                    const int* __restrict__ done, int ndc,
-                                                   const double* __restrict__ r, double* __restrict__ z) {
+                                                   const double* __restrict__ r, Tz* __restrict__ z) {
   constexpr int NM = ND * MAPLE_ISL_MAXN;
   __shared__ double rv[NM];
   const int s = blockIdx.x, j = blockIdx.y;
@@ -274,7 +274,8 @@ __global__ __launch_bounds__(128) void k_isl_apply(const Island* __restrict__ is
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     double acc = 0.0;
     for (int k = 0; k < n; ++k) acc = fma(A[t * n + k], rv[k], acc);
-    z[(long long)(j * ND + t % ND) * Ntot + I.nodes[t / ND]] += acc;
+    Tz& zt = z[(long long)(j * ND + t % ND) * Ntot + I.nodes[t / ND]];
+    zt = (Tz)((double)zt + acc);
   }
 }
 
@@ -301,10 +302,22 @@ void island_setup(int nd, const LevelCell* cells, Island* isl, int n, const doub
   else k_isl_setup<1><<<n, 256, 0, s>>>(cells, isl, Ke, rho, inv, valid);
 }
 
-void island_apply(int nd, int ncase, const Island* isl, const int* valid, int n, const double* inv, long long Ntot,
-                  const int* done, int ndc, const double* r, double* z, hipStream_t s) {
+template <typename Tz>
+void island_apply_t(int nd, int ncase, const Island* isl, const int* valid, int n, const double* inv, long long Ntot,
+                    const int* done, int ndc, const double* r, Tz* z, hipStream_t s) {
   if (n <= 0) return;
   dim3 grid(n, ncase);
-  if (nd == 3) k_isl_apply<3><<<grid, 128, 0, s>>>(isl, valid, inv, Ntot, done, ndc, r, z);
-  else k_isl_apply<1><<<grid, 128, 0, s>>>(isl, valid, inv, Ntot, done, ndc, r, z);
+  if (nd == 3) k_isl_apply<3, Tz><<<grid, 128, 0, s>>>(isl, valid, inv, Ntot, done, ndc, r, z);
+  else k_isl_apply<1, Tz><<<grid, 128, 0, s>>>(isl, valid, inv, Ntot, done, ndc, r, z);
+}
+
+void island_apply(int nd, int ncase, const Island* isl, const int* valid, int n, const double* inv, long long Ntot,
+                  const int* done, int ndc, const double* r, double* z, hipStream_t s) {
+  island_apply_t<double>(nd, ncase, isl, valid, n, inv, Ntot, done, ndc, r, z, s);
+}
+
+// the mixed form's FP32 preconditioned residual
+void island_apply(int nd, int ncase, const Island* isl, const int* valid, int n, const double* inv, long long Ntot,
+                  const int* done, int ndc, const double* r, float* z, hipStream_t s) {
+  island_apply_t<float>(nd, ncase, isl, valid, n, inv, Ntot, done, ndc, r, z, s);
 }

@@ -161,6 +161,13 @@ void pcg_pick_start(const Launch& L, int ncells, int nd, int ncase, bool have_pr
                     double* u, double* r, const double* us, const double* rs, const double* f, hipStream_t s);
 // p = z + beta p
 void pcg_update_p(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* z, double* p, hipStream_t s);
+// mixed form, (nd, ncase) = (3, 6) or (1, 3) only (else -1, nothing launched): the
+// update also writes r32 = (float) r and, if zold is not null, partial2 = (r, zold)
+int pcg_update_ur_mixed(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* p, const double* q,
+                        double* u, double* r, float* r32, const float* zold, double* partial, double* partial2,
+                        hipStream_t s);
+int pcg_update_p32(const Launch& L, int nd, int ncase, const CaseScalars* sc, const float* z, double* p,
+                   hipStream_t s);
 // per (cell, case) scalar steps
 void pcg_init_scalars(int ncells, int ncase, double rtol, const double* atol2, CaseScalars* sc, int* done_flags,
                       hipStream_t s);
@@ -223,3 +230,5 @@ void island_setup(int nd, const LevelCell* cells, Island* isl, int n, const doub
                   double* inv, int* valid, hipStream_t s);
 void island_apply(int nd, int ncase, const Island* isl, const int* valid, int n, const double* inv, long long Ntot,
                   const int* done, int ndc, const double* r, double* z, hipStream_t s);
+void island_apply(int nd, int ncase, const Island* isl, const int* valid, int n, const double* inv, long long Ntot,
+                  const int* done, int ndc, const double* r, float* z, hipStream_t s);

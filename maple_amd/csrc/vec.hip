@@ -184,6 +184,52 @@ __global__ __launch_bounds__(256) void k_update_ur(Launch L, int nd, int ncase, 
   }
 }
 
+// The mixed form's update: u += alpha p, r -= alpha q as k_update_ur, and in the same
+// pass the FP32 copy of the new r (the FP32 preconditioner's input) and, when the
+// previous preconditioned residual was FP32 (zold != nullptr), the flexible-CG dot
+// (r_new, z_old) into partial2.  Saves the separate copy and dot passes.
+template <int ND, int NC>
+__global__ __launch_bounds__(256) void k_update_ur_mixed(Launch L, const CaseScalars* __restrict__ sc,
+                                                         const double* __restrict__ p, const double* __restrict__ q,
+                                                         double* __restrict__ u, double* __restrict__ r,
+                                                         float* __restrict__ r32, const float* __restrict__ zold,
+                                                         double* __restrict__ partial, double* __restrict__ partial2) {
+  int ci; long long gid = 0;
+  const bool ok = vnode(L, ci, gid);
+  double v[NC], w[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    v[j] = w[j] = 0.0;
+    const double al = sc[ci * NC + j].alpha;
+    if (ok && al != 0.0) {   // a converged case keeps u, r and r32 untouched
+#pragma unroll
+      for (int i = 0; i < ND; ++i) {
+        const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
+        u[idx] = fma(al, p[idx], u[idx]);
+        const double rn = fma(-al, q[idx], r[idx]);
+        r[idx] = rn;
+        r32[idx] = (float)rn;
+        v[j] += rn * rn;
+        if (zold != nullptr) w[j] += rn * (double)zold[idx];
+      }
+    }
+  }
+  block_sum<NC>(v, partial + (long long)lbid() * MAPLE_MAX_CASES);
+  if (zold != nullptr) block_sum<NC>(w, partial2 + (long long)lbid() * MAPLE_MAX_CASES);
+}
+
+int pcg_update_ur_mixed(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* p, const double* q,
+                        double* u, double* r, float* r32, const float* zold, double* partial, double* partial2,
+                        hipStream_t s) {
+  if (nd == 3 && ncase == 6)
+    k_update_ur_mixed<3, 6><<<L.nblocks, 256, 0, s>>>(L, sc, p, q, u, r, r32, zold, partial, partial2);
+  else if (nd == 1 && ncase == 3)
+    k_update_ur_mixed<1, 3><<<L.nblocks, 256, 0, s>>>(L, sc, p, q, u, r, r32, zold, partial, partial2);
+  else
+    return -1;
+  return 0;
+}
+
 void pcg_update_ur(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* p, const double* q,
                    double* u, double* r, double* partial, hipStream_t s) {
   if (nd == 3 && ncase == 6) k_update_ur_t<3, 6><<<L.nblocks, 256, 0, s>>>(L, sc, p, q, u, r, partial);
@@ -191,9 +237,9 @@ void pcg_update_ur(const Launch& L, int nd, int ncase, const CaseScalars* sc, co
   else k_update_ur<<<L.nblocks, 256, 0, s>>>(L, nd, ncase, sc, p, q, u, r, partial);
 }
 
-template <int ND, int NC>
+template <int ND, int NC, typename Tz>
 __global__ __launch_bounds__(256) void k_update_p_t(Launch L, const CaseScalars* __restrict__ sc,
-                                                    const double* __restrict__ z, double* __restrict__ p) {
+                                                    const Tz* __restrict__ z, double* __restrict__ p) {
   int ci; long long gid;
   if (!vnode(L, ci, gid)) return;
 #pragma unroll
@@ -203,7 +249,7 @@ __global__ __launch_bounds__(256) void k_update_p_t(Launch L, const CaseScalars*
 #pragma unroll
     for (int i = 0; i < ND; ++i) {
       const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
-      p[idx] = fma(be, p[idx], z[idx]);
+      p[idx] = fma(be, p[idx], (double)z[idx]);
     }
   }
 }
@@ -224,9 +270,18 @@ __global__ __launch_bounds__(256) void k_update_p(Launch L, int nd, int ncase, c
 
 void pcg_update_p(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* z, double* p,
                   hipStream_t s) {
-  if (nd == 3 && ncase == 6) k_update_p_t<3, 6><<<L.nblocks, 256, 0, s>>>(L, sc, z, p);
-  else if (nd == 1 && ncase == 3) k_update_p_t<1, 3><<<L.nblocks, 256, 0, s>>>(L, sc, z, p);
+  if (nd == 3 && ncase == 6) k_update_p_t<3, 6, double><<<L.nblocks, 256, 0, s>>>(L, sc, z, p);
+  else if (nd == 1 && ncase == 3) k_update_p_t<1, 3, double><<<L.nblocks, 256, 0, s>>>(L, sc, z, p);
   else k_update_p<<<L.nblocks, 256, 0, s>>>(L, nd, ncase, sc, z, p);
+}
+
+// p = z + beta p with the FP32 preconditioned residual of the mixed form
+int pcg_update_p32(const Launch& L, int nd, int ncase, const CaseScalars* sc, const float* z, double* p,
+                   hipStream_t s) {
+  if (nd == 3 && ncase == 6) k_update_p_t<3, 6, float><<<L.nblocks, 256, 0, s>>>(L, sc, z, p);
+  else if (nd == 1 && ncase == 3) k_update_p_t<1, 3, float><<<L.nblocks, 256, 0, s>>>(L, sc, z, p);
+  else return -1;
+  return 0;
 }
 
 __global__ void k_pick_src(int n, int have_prev, int have_start, CaseScalars* __restrict__ sc) {
