@@ -394,8 +394,62 @@ __global__ __launch_bounds__(256) void k_prolong_add(const LevelCell* __restrict
   uf[(long long)blockIdx.y * Ntot_f + gid] += acc;
 }
 
+// One thread per fine node for every (case, component) plane: the coarse
+// neighbours and weights are found once; the z parity varies across a wavefront,
+// so both coarse z neighbours are read with weights (1, 0) or (1/2, 1/2) instead of
+// branching (x and y parities are uniform over a z-pencil).
+template <int ND, int NC, typename T>
+__global__ __launch_bounds__(256) void k_prolong_add_t(const LevelCell* __restrict__ fcells,
+                                                       const int* __restrict__ blk_start, int ncells, long long Ntot_f,
+                                                       const LevelCell* __restrict__ ccells, long long Ntot_c,
+                                                       const T* __restrict__ ec, T* __restrict__ uf,
+                                                       const int* __restrict__ done, int ndc) {
+  LevelCell c; int ci, x, y, z; long long gid;
+  int kc;
+  if (!node_of(fcells, blk_start, ncells, lbid(), c, ci, x, y, z, gid, &kc)) return;
+  const LevelCell cc = ccells[kc];
+  const int bx = x >> 1, by = y >> 1, bz = z >> 1;
+  const int rx = x & 1, ry = y & 1, rz = z & 1;
+  const int cx[2] = {bx, rx ? wrap(bx + 1, cc.nx) : bx};
+  const int cy[2] = {by, ry ? wrap(by + 1, cc.ny) : by};
+  const int cz1 = rz ? wrap(bz + 1, cc.nz) : bz;
+  const T wxy = (T)((rx ? 0.5 : 1.0) * (ry ? 0.5 : 1.0));
+  const T wz0 = rz ? T(0.5) : T(1), wz1 = rz ? T(0.5) : T(0);
+  long long cn[4];
+  int nk = 0;
+#pragma unroll
+  for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+    for (int oy = 0; oy < 2; ++oy) {
+      if ((ox && !rx) || (oy && !ry)) continue;
+      cn[nk++] = cc.off + ((long long)cx[ox] * cc.ny + cy[oy]) * cc.nz;
+    }
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    if (case_done(done, ndc, ci, j)) continue;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const T* __restrict__ e = ec + (long long)(j * ND + d) * Ntot_c;
+      T acc = T(0);
+      for (int k = 0; k < nk; ++k) acc += wz0 * e[cn[k] + bz] + wz1 * e[cn[k] + cz1];
+      uf[(long long)(j * ND + d) * Ntot_f + gid] += wxy * acc;
+    }
+  }
+}
+
+// Launches of 4096 or more blocks (level 0 of a batch of four 64^3 cells) use the node form; smaller ones
+// (one cell's coarse levels) the plane-per-thread form, whose extra blocks hide latency.
 template <typename T>
 void prolong_add(const Launch& Lf, const Launch& Lc, int nd, int ncase, const T* ec, T* uf, hipStream_t s) {
+  const bool node_form = Lf.nblocks >= 4096;
+  if (node_form && nd == 3 && ncase == 6) {
+    k_prolong_add_t<3, 6, T><<<Lf.nblocks, 256, 0, s>>>(Lf.cells, Lf.blk_start, Lf.ncells, Lf.Ntot, Lc.cells, Lc.Ntot, ec, uf, Lf.done, Lf.ndc);
+    return;
+  }
+  if (node_form && nd == 1 && ncase == 3) {
+    k_prolong_add_t<1, 3, T><<<Lf.nblocks, 256, 0, s>>>(Lf.cells, Lf.blk_start, Lf.ncells, Lf.Ntot, Lc.cells, Lc.Ntot, ec, uf, Lf.done, Lf.ndc);
+    return;
+  }
   dim3 grid(Lf.nblocks, nd * ncase);
   k_prolong_add<T><<<grid, 256, 0, s>>>(Lf.cells, Lf.blk_start, Lf.ncells, Lf.Ntot, Lc.cells, Lc.Ntot, ec, uf, Lf.done, Lf.ndc, nd);
 }
