@@ -78,6 +78,13 @@ int maple_set_element_matrices(maple_ctx* ctx, int cell, const double* Ke24, con
  * EBE_GMG.py:858). */
 int maple_set_solver(maple_ctx* ctx, double omega, int nsmooth, int warm_start);
 
+/* Jacobi sweeps of the V-cycle before (pre >= 1, the first from a zero start)
+ * and after (post >= 0) the coarse-grid correction; maple_set_solver sets both
+ * to nsmooth.  Unequal counts make the V-cycle nonsymmetric: use flexible CG
+ * (maple_set_cg(ctx, 1); the mixed form's default).  The reference's recursion
+ * uses smooth_iter per level (EBE_GMG.py:886-984). */
+int maple_set_sweeps(maple_ctx* ctx, int pre, int post);
+
 /* Smoother of the V-cycle: MAPLE_SMOOTH_JACOBI (weighted point Jacobi; the
  * reference's smooth_jacobi, EBE_GMG.py:395-411) or MAPLE_SMOOTH_L1_JACOBI
  * (the diagonal replaced by the absolute row sum, convergent for every

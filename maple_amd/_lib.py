@@ -44,6 +44,7 @@ SIGNATURES = {
     "maple_set_element_matrices": (_I, [_P, _I, _DP, _DP, _DP, _DP, _DP, _DP]),
     "maple_set_solver": (_I, [_P, _D, _I, _I]),
     "maple_set_smoother": (_I, [_P, _I]),
+    "maple_set_sweeps": (_I, [_P, _I, _I]),
     "maple_set_islands": (_I, [_P, _I, _D]),
     "maple_num_islands": (_I, [_P]),
     "maple_set_cg": (_I, [_P, _I]),

@@ -127,6 +127,10 @@ class CellBatch:
         """Solve elastic and thermal at the same time (second stream and host thread) or in turn."""
         L.check(self.lib.maple_set_concurrency(self._h, int(bool(on))))
 
+    def set_sweeps(self, pre: int = 2, post: int = 2):
+        """Jacobi sweeps before / after the coarse correction (unequal: use flexible CG)."""
+        L.check(self.lib.maple_set_sweeps(self._h, int(pre), int(post)))
+
     def set_islands(self, on: bool = True, threshold: float = 1e-4):
         """Island correction of the preconditioner: small solid islands (rho > threshold,
         at most 16 elements and 32 nodes) are solved exactly beside the V-cycle."""

@@ -123,7 +123,8 @@ struct maple_ctx {
   Phys ph[2];
   double* partial = nullptr;
   double omega = 1.6;   // l1-Jacobi: the spectrum of Dinv K lies in (0, 1], any omega < 2 converges
-  int nsmooth = 2;
+  int nsmooth = 2;          // Jacobi sweeps before the coarse correction (the first from zero)
+  int npost = 2;            // and after it
   int warm = 0;
   int smoother = MAPLE_SMOOTH_L1_JACOBI;
   int flexible = -1;   // -1: flexible CG in the mixed form only; 2: block CG
@@ -467,7 +468,7 @@ T* vcycle_t(maple_ctx* c, Phys& P, VC<T>& V, int l, const T* b, hipStream_t s) {
   restrict_pt<T>(P.ln[l + 1], P.ln[l], P.nd, P.nc, V.R[l], V.B[l + 1], s);
   T* e = vcycle_t<T>(c, P, V, l + 1, V.B[l + 1], s);
   prolong_add<T>(P.ln[l], P.ln[l + 1], P.nd, P.nc, e, x, s);
-  for (int k = 0; k < c->nsmooth; ++k) {
+  for (int k = 0; k < c->npost; ++k) {
     apply_t<T>(c, P, V, l, 3, x, b, t, s);
     std::swap(x, t);
   }
@@ -994,6 +995,7 @@ int maple_set_solver(maple_ctx* c, double omega, int nsmooth, int warm_start) {
   if (nsmooth < 1 || nsmooth > 16) return fail(MAPLE_EINVAL, "nsmooth must lie in [1, 16]");
   c->omega = omega;
   c->nsmooth = nsmooth;
+  c->npost = nsmooth;
   c->warm = warm_start ? 1 : 0;
   return 0;
 }
@@ -1081,6 +1083,14 @@ int maple_set_islands(maple_ctx* c, int on, double threshold) {
 }
 
 int maple_num_islands(const maple_ctx* c) { return c ? c->nisl : 0; }
+
+int maple_set_sweeps(maple_ctx* c, int pre, int post) {
+  if (!c) return fail(MAPLE_EINVAL, "null context");
+  if (pre < 1 || pre > 16 || post < 0 || post > 16) return fail(MAPLE_EINVAL, "sweeps: pre in [1, 16], post in [0, 16]");
+  c->nsmooth = pre;
+  c->npost = post;
+  return 0;
+}
 
 int maple_set_smoother(maple_ctx* c, int kind) {
   if (!c) return fail(MAPLE_EINVAL, "null context");

@@ -289,12 +289,56 @@ __global__ __launch_bounds__(256) void k_dense_apply(const LevelCell* __restrict
   if (threadIdx.x < ncase) x[(long long)(threadIdx.x * ND + i) * Ntot + dc.voff + m] = (T)red[threadIdx.x];
 }
 
+// The same product with RB rows per block: each thread reads a right-hand-side
+// column entry once for RB rows (the one-row form reads the whole right-hand side
+// once per row, several times the bytes of the matrix itself).
+template <int ND, int RB, typename T>
+__global__ __launch_bounds__(256) void k_dense_apply_rb(const LevelCell* __restrict__ cells,
+                                                        const DenseCell* __restrict__ dcells, long long Ntot,
+                                                        int ncase, const double* __restrict__ Ainv,
+                                                        const T* __restrict__ b, T* __restrict__ x,
+                                                        const int* __restrict__ done) {
+  const int ci = cells[blockIdx.y].id;
+  const DenseCell dc = dcells[ci];
+  const int row0 = blockIdx.x * RB;
+  if (row0 >= dc.n) return;
+  if (cell_done(done, ncase, ci)) return;
+  const int nr = min(RB, dc.n - row0);
+  double acc[RB * MAPLE_MAX_CASES];
+#pragma unroll
+  for (int q = 0; q < RB * MAPLE_MAX_CASES; ++q) acc[q] = 0.0;
+  const double* __restrict__ A0 = Ainv + dc.aoff + (long long)row0 * dc.n;
+  for (int col = threadIdx.x; col < dc.n; col += 256) {
+    const int m = col / ND, i = col - m * ND;
+    double bv[MAPLE_MAX_CASES];
+#pragma unroll
+    for (int j = 0; j < MAPLE_MAX_CASES; ++j)
+      bv[j] = j < ncase ? (double)b[(long long)(j * ND + i) * Ntot + dc.voff + m] : 0.0;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      if (r >= nr) break;   // uniform over the block
+      const double a = A0[(long long)r * dc.n + col];
+#pragma unroll
+      for (int j = 0; j < MAPLE_MAX_CASES; ++j) acc[r * MAPLE_MAX_CASES + j] = fma(a, bv[j], acc[r * MAPLE_MAX_CASES + j]);
+    }
+  }
+  __shared__ double red[RB * MAPLE_MAX_CASES];
+  block_sum<RB * MAPLE_MAX_CASES>(acc, red);
+  if ((int)threadIdx.x < nr * ncase) {
+    const int r = threadIdx.x / ncase, j = threadIdx.x - r * ncase;
+    const int row = row0 + r;
+    const int m = row / ND, i = row - m * ND;
+    x[(long long)(j * ND + i) * Ntot + dc.voff + m] = (T)red[r * MAPLE_MAX_CASES + j];
+  }
+}
+
 template <typename T>
 void dense_apply(const Launch& Lc, const DenseCell* dcells, int max_n, int nd, int ncase, const double* Ainv,
                  const T* b, T* x, hipStream_t s) {
-  dim3 grid(max_n, Lc.ncells);
-  if (nd == 3) k_dense_apply<3, T><<<grid, 256, 0, s>>>(Lc.cells, dcells, Lc.Ntot, ncase, Ainv, b, x, Lc.done);
-  else k_dense_apply<1, T><<<grid, 256, 0, s>>>(Lc.cells, dcells, Lc.Ntot, ncase, Ainv, b, x, Lc.done);
+  constexpr int RB = 8;
+  dim3 grid((max_n + RB - 1) / RB, Lc.ncells);
+  if (nd == 3) k_dense_apply_rb<3, RB, T><<<grid, 256, 0, s>>>(Lc.cells, dcells, Lc.Ntot, ncase, Ainv, b, x, Lc.done);
+  else k_dense_apply_rb<1, RB, T><<<grid, 256, 0, s>>>(Lc.cells, dcells, Lc.Ntot, ncase, Ainv, b, x, Lc.done);
 }
 template void dense_apply<double>(const Launch&, const DenseCell*, int, int, int, const double*, const double*, double*, hipStream_t);
 template void dense_apply<float>(const Launch&, const DenseCell*, int, int, int, const double*, const float*, float*, hipStream_t);
