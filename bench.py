@@ -16,7 +16,8 @@ Forms (one JSON line; ``value`` is the headline form):
                FP64 CG around an FP32 multigrid preconditioner with the FP64
                island correction (small solid islands solved exactly beside the
                V-cycle), each solve starting from the best of zero, the previous
-               iterate's fields and the learned network's prediction.
+               iterate's fields and those fields plus the change the learned
+               network predicts between the two densities.
                value = cells x iterations / s.
 * optional (--forms): single-cell and batched ablations (``fp64-warm``,
   ``fp64-islands``, ``fp64-block``, ``mixed-warm``, ``mixed-warm-block``,
@@ -84,7 +85,7 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
             start = None
             if learned:
                 from maple_amd import learned as ml
-                start = ml.LearnedStart(cells, ml.UNet.load(max_batch=min(16, per), device=dev))
+                start = ml.LearnedStart(cells, ml.UNet.load(max_batch=min(16, per), device=dev), mode="delta")
             drv = maple_amd.OCDriver(cells, OBJ, 0.25, move=0.2, xmin=1e-3, filter_k=3, rtol=1e-6, maxit=maxit,
                                      start=start)
             out.append((cells, drv))

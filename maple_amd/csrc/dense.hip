@@ -6,7 +6,8 @@
 // mean diagonal); for a right-hand side orthogonal to Z the solution of
 // A' x = b is the minimum-norm solution of A x = b.  A'^-1 is formed once per
 // density field by an in-place blocked Gauss-Jordan sweep (64 x 64 tiles, no
-// pivoting, valid for SPD) and each V-cycle applies it as a dense mat-vec over
+// pivoting, valid for SPD; the rank-64 updates of every step, 2 n^3 of its flops,
+// on FP64 MFMA) and each V-cycle applies it as a dense mat-vec over
 // all load cases, which keeps the coarsest solve a bandwidth-bound GEMV
 // instead of two latency-bound triangular solves.
 #include "kernels.h"
@@ -208,20 +209,68 @@ __global__ __launch_bounds__(256) void k_gj_row(const DenseCell* __restrict__ dc
   store_tile(dc, A, k, j, acc, 1.0, false);
 }
 
-// K3: A_ij -= A_ik A_kj (i != k, j != k), A_kj already updated
+// K3 on the matrix cores: A_ij -= A_ik A_kj (i != k, j != k; A_kj already updated),
+// a 64 x 64 x 64 tile product with v_mfma_f64_16x16x4 (FP64 throughout; the sums
+// differ from scalar FMAs only in their order).  The operand tiles are staged in LDS
+// with rows padded by one double (the A fragments read 16 rows of one column);
+// wave w computes the 32 x 32 quarter (w / 2, w % 2) as 2 x 2 blocks of 16 x 16.
+// Fragments of the 16x16x4 product: A (16 x 4), lane l holds row l % 16, column
+// l / 16; B (4 x 16), lane l holds row l / 16, column l % 16; D (16 x 16), lane l
+// holds column l % 16, rows l / 16 + 4 r in its four values (checked on gfx950).
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void load_tile_pad(const DenseCell& dc, const double* __restrict__ A, int ti, int tj,
+                                              double (*L)[TB + 1]) {
+  const int nr = tile_rows(dc, ti), nc = tile_rows(dc, tj);
+  for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
+    const int r = e / TB, cc = e - r * TB;
+    L[r][cc] = (r < nr && cc < nc) ? A[dc.aoff + (long long)(ti * TB + r) * dc.n + tj * TB + cc] : 0.0;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_gj_update(const DenseCell* __restrict__ dcells, int k,
                                                    double* __restrict__ A) {
   const int ci = blockIdx.z, i = blockIdx.y, j = blockIdx.x;
   const DenseCell dc = dcells[ci];
   if (k >= dc.nt || i >= dc.nt || j >= dc.nt || i == k || j == k) return;
-  __shared__ double X[TB][TB];
-  __shared__ double Y[TB][TB];
-  load_tile(dc, A, i, k, X);
-  load_tile(dc, A, k, j, Y);
+  __shared__ double X[TB][TB + 1];
+  __shared__ double Y[TB][TB + 1];
+  load_tile_pad(dc, A, i, k, X);
+  load_tile_pad(dc, A, k, j, Y);
   __syncthreads();
-  double acc[4][4];
-  tile_mm(X, Y, acc);
-  store_tile(dc, A, i, j, acc, -1.0, true);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
+  const int li = l & 15, lk = l >> 4;
+  dbl4 acc[2][2];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int kk = 0; kk < TB; kk += 4) {
+    double a[2], b[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) a[rb] = X[r0 + rb * 16 + li][kk + lk];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) b[cb] = Y[kk + lk][c0 + cb * 16 + li];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+  }
+  const int nr = tile_rows(dc, i), nc = tile_rows(dc, j);
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int row = r0 + rb * 16 + lk + 4 * rr, col = c0 + cb * 16 + li;
+        if (row < nr && col < nc) {
+          double* p = A + dc.aoff + (long long)(i * TB + row) * dc.n + j * TB + col;
+          *p -= acc[rb][cb][rr];
+        }
+      }
 }
 
 // K4: A_ik <- -A_ik P (i != k), A_kk <- P

@@ -305,10 +305,22 @@ class LearnedStart:
     Cells of the network's resolution get the prediction on their own grid; cells
     of twice that resolution get it on their level-1 grid, prolongated to level 0
     by the multigrid's periodic trilinear P.  Other cells get no candidate (zero).
+
+    mode "direct": the candidate is the prediction itself.  mode "delta": from the
+    second call on, the candidate is the previous solve's fields plus the change
+    the network predicts between the previous density and this one,
+    u_prev + N(rho_t) - N(rho_{t-1}); the network's systematic error is mostly the
+    same for two neighbouring densities and cancels in the difference, while the
+    fields carry everything it cannot resolve.  The solve still starts from the best
+    of zero, the previous fields and this candidate.
     """
 
-    def __init__(self, cells: CellBatch, net: UNet):
+    def __init__(self, cells: CellBatch, net: UNet, mode: str = "direct"):
+        if mode not in ("direct", "delta"):
+            raise ValueError("mode must be 'direct' or 'delta'")
         self.cells, self.net = cells, net
+        self.mode = mode
+        self.prev = None
         n = net.n
         self.l0 = [i for i, d in enumerate(cells.dims) if d == (n, n, n)]
         self.l1 = [i for i, d in enumerate(cells.dims) if d == (2 * n, 2 * n, 2 * n)]
@@ -330,6 +342,13 @@ class LearnedStart:
         if not self.l0 and not self.l1:
             return
         el0, th0 = self.predict(rho)
+        prev, self.prev = self.prev, (el0, th0)
+        if self.mode == "delta" and prev is not None:
+            if "elastic" in physics:
+                self.cells.set_start("elastic", self.cells.fields("elastic") + (el0 - prev[0]))
+            if "thermal" in physics:
+                self.cells.set_start("thermal", self.cells.fields("thermal") + (th0 - prev[1]))
+            return
         if "elastic" in physics:
             self.cells.set_start("elastic", el0)
         if "thermal" in physics:

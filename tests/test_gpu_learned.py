@@ -125,6 +125,44 @@ def test_start_selection_never_worse_than_zero():
     assert torch.linalg.norm(CH2 - CH0) <= 1e-9 * torch.linalg.norm(CH0)
 
 
+def test_delta_start_is_fields_plus_predicted_change():
+    """LearnedStart(mode="delta"): the first call supplies the prediction; later calls
+    supply the previous solve's fields plus the predicted change between the two
+    densities, and the solve still converges to the same C_H as a zero start."""
+    import maple_amd
+    from maple_amd import learned as ml
+    from oracle import unet as ounet
+    r = 32
+    rng = np.random.default_rng(5)
+    theta = ounet.bf16_round(ounet.init_params(2)).astype(np.float32)
+    net = ml.UNet(n=16, max_batch=1, device=DEV, theta=theta)
+    cb = maple_amd.CellBatch([r], nlevels=3, device=DEV)
+    ls = ml.LearnedStart(cb, net, mode="delta")
+    rho1 = torch.from_numpy(1e-9 + (1 - 1e-9) * rng.uniform(0.05, 1, r ** 3) ** 3).to(DEV)
+    rho2 = (rho1 * torch.from_numpy(rng.uniform(0.9, 1.1, r ** 3)).to(DEV)).clamp(1e-9, 1.0)
+    ls(rho1, ("elastic",))
+    cb.homogenize(rho1, ("elastic",), rtol=1e-8)
+    u1 = cb.fields("elastic")
+    p1, _ = ls.predict(rho1)
+    p2, _ = ls.predict(rho2)
+    ls(rho2, ("elastic",))
+    # the candidate the solve sees: the converged fields of rho1 plus p2 - p1
+    cand = u1 + (p2 - p1)
+    cb2 = maple_amd.CellBatch([r], nlevels=3, device=DEV)
+    cb2.set_start("elastic", cand)
+    CHa, _, _ = cb2.homogenize(rho2, ("elastic",), rtol=1e-8)
+    CHb, _, _ = cb.homogenize(rho2, ("elastic",), rtol=1e-8)
+    src_a, src_b = cb2.start_sources("elastic"), cb.start_sources("elastic")
+    np.testing.assert_array_equal(src_a, src_b)
+    assert torch.linalg.norm(CHa - CHb) <= 1e-9 * torch.linalg.norm(CHb)
+    cz = maple_amd.CellBatch([r], nlevels=3, device=DEV)
+    CHz, _, _ = cz.homogenize(rho2, ("elastic",), rtol=1e-10)
+    assert torch.linalg.norm(CHb - CHz) <= 1e-6 * torch.linalg.norm(CHz)
+    for c in (cb, cb2, cz):
+        c.close()
+    net.close()
+
+
 def test_prolong_matches_oracle():
     import maple_amd
     from oracle import mg
