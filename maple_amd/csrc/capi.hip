@@ -118,6 +118,7 @@ struct Timer {
 
 struct maple_ctx {
   int device = 0, ncells = 0, nlevels = 0, prec = 0;
+  bool all_nz64 = false;   // every cell has nz = 64 (level-0 operator variant 8 applies)
   std::vector<int> dims;
   std::vector<Level> lv;
   Phys ph[2];
@@ -361,9 +362,11 @@ void timed_end(maple_ctx* c, int kind, hipStream_t s) {
 // kernels, or the node launch of the node-block kernels (elastic, variants 2-3).
 // full: every cell and case (no convergence flags).
 Launch op_launch(maple_ctx* c, const Phys& P, bool full) {
-  const int v = c->lv[0].fine.var;
+  int v = c->lv[0].fine.var;
+  if (v == 8 && !c->all_nz64) v = 5;                     // the lane-rotation form needs nz = 64
   const int var = P.nd == 3 ? v : (v == 1 ? 1 : 0);   // thermal: element rows only
-  Launch L = var >= 6 ? (full ? c->lv[0].quad : P.lq)
+  Launch L = var == 8 ? (full ? c->lv[0].pair : P.lx)
+           : var >= 6 ? (full ? c->lv[0].quad : P.lq)
            : var >= 4 ? (full ? c->lv[0].pair : P.lx)
            : var >= 2 ? (full ? c->lv[0].node : P.ln[0])
                       : (full ? c->lv[0].fine : P.lf);
@@ -379,7 +382,9 @@ void apply_fine64(maple_ctx* c, Phys& P, int mode, const double* rho, const doub
   L.work = c->timing ? c->work_d + kind : nullptr;   // the launch counts its own work
   const double* dinv = P.v64.dinv.empty() ? nullptr : P.v64.dinv[0];
   timed_begin(c, kind, s);
-  if (P.nd == 3 && L.var >= 4)
+  if (P.nd == 3 && L.var == 8)
+    fine_apply_nodeb64<double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
+  else if (P.nd == 3 && L.var >= 4)
     fine_apply_nodeb<double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
   else if (P.nd == 3 && L.var >= 2)
     fine_apply_node<double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
@@ -398,7 +403,9 @@ void apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const T* u, const
     Launch L = op_launch(c, P, false);
     L.work = c->timing && kind >= 0 ? c->work_d + kind : nullptr;
     if (kind >= 0) timed_begin(c, kind, s);
-    if (P.nd == 3 && L.var >= 4)
+    if (P.nd == 3 && L.var == 8)
+      fine_apply_nodeb64<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+    else if (P.nd == 3 && L.var >= 4)
       fine_apply_nodeb<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
     else if (P.nd == 3 && L.var >= 2)
       fine_apply_node<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
@@ -871,6 +878,8 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
   c->nlevels = nlevels;
   c->prec = precision;
   c->dims.assign(dims_host, dims_host + 3 * ncells);
+  c->all_nz64 = true;
+  for (int i = 0; i < ncells; ++i) c->all_nz64 = c->all_nz64 && dims_host[3 * i + 2] == 64;
   int rc = set_device(c);
   auto bail = [&](int code) {
     std::string keep = g_err;
@@ -1003,7 +1012,7 @@ int maple_set_solver(maple_ctx* c, double omega, int nsmooth, int warm_start) {
 int maple_set_variant(maple_ctx* c, int kernel, int variant) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
   if (kernel == MAPLE_KERNEL_FINE_OPERATOR) {
-    if (variant < 0 || variant > 7) return fail(MAPLE_EINVAL, "unknown variant");
+    if (variant < 0 || variant > 8) return fail(MAPLE_EINVAL, "unknown variant");
     c->lv[0].fine.var = variant;
     for (Phys& P : c->ph) P.lf.var = variant;
     return 0;

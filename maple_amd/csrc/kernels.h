@@ -21,7 +21,8 @@ struct Launch {
   // kernel variant of the level-0 elastic operator, picked by the tuner: element rows
   // per pencil lane (0: neighbour values in registers, 1: staged element by element)
   // or node blocks (2: six load cases per thread, 3: three; 4, 5: the same with two
-  // x-adjacent nodes per thread; 6, 7: 2x2 nodes in x and z, three / two cases)
+  // x-adjacent nodes per thread; 6, 7: 2x2 nodes in x and z, three / two cases;
+  // 8: as 5 with z neighbours taken from the neighbouring lanes, cells with nz = 64)
   int var = 0;
   // level-0 operator timing: when set, the launch adds the elements x load cases it
   // processes (listed cells, cases not converged) to *work
@@ -59,6 +60,11 @@ void fine_apply_node(const Launch& L, int mode, int ncase, const T* Ke, const T*
 template <typename T>
 void fine_apply_nodeb(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u,
                       const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s);
+// Variant 8: as variant 5 for batches whose cells all have nz = 64 (a wavefront is
+// one z-pencil; z neighbours come from the neighbouring lanes).  L: the pair launch.
+template <typename T>
+void fine_apply_nodeb64(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u,
+                        const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s);
 void fine_rhs(const Launch& L, int nd, int ncase, const double* Fe, const double* rho, double* F, hipStream_t s);
 template <typename T>
 void fine_dinv(const Launch& L, int nd, bool l1, const double* Ke, const double* rho, T* dinv, hipStream_t s);

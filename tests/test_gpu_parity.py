@@ -51,8 +51,8 @@ def test_operator_and_rhs(r, nlev, physics):
     assert np.abs(F - Fo.transpose(2, 1, 0)).max() <= 1e-13 * np.abs(Fo).max()
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7])
-@pytest.mark.parametrize("r,nlev,ncase", [(16, 3, 6), (24, 3, 4), (72, 4, 5)])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("r,nlev,ncase", [(16, 3, 6), (24, 3, 4), (72, 4, 5), (64, 4, 6)])
 def test_operator_variants(r, nlev, ncase, variant):
     """Every level-0 elastic operator variant (element rows per pencil lane, staged;
     node blocks with six or three load cases per thread, one, 2x1 or 2x2 nodes
@@ -72,7 +72,7 @@ def test_operator_variants(r, nlev, ncase, variant):
     cb.close()
 
 
-@pytest.mark.parametrize("variant", [2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_homogenize_node_variants(variant, precision):
     """The node-block operator inside the whole solve (CG matvec, residuals, and the
@@ -90,6 +90,27 @@ def test_homogenize_node_variants(variant, precision):
     assert np.linalg.norm(CH[0].cpu().numpy() - CHo) <= 1e-9 * np.linalg.norm(CHo)
     assert np.linalg.norm(kap[0].cpu().numpy() - ko) <= 1e-9 * np.linalg.norm(ko)
     cb.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lane_rotation_variant_64(precision):
+    """Variant 8 (z neighbours from the neighbouring lanes, nz = 64 only) inside the
+    whole 64^3 solve agrees with variant 5 (checked against the oracle above): the
+    same operator up to rounding, so the same C_H and CG counts."""
+    import maple_amd
+    r, nlev = 64, 4
+    print(f"[variant 8] {precision}", flush=True)
+    rho = _rho(r, 21)
+    out = []
+    for v in (5, 8):
+        cb = maple_amd.CellBatch([r], nlevels=nlev, device=DEV, precision=precision)
+        cb.set_variant(maple_amd._lib.MAPLE_KERNEL_FINE_OPERATOR, v)
+        CH, kap, it = cb.homogenize(_t(rho), rtol=1e-8, maxit=2000)
+        out.append((CH[0].cpu().numpy(), kap[0].cpu().numpy(), it[0].astype(int)))
+        cb.close()
+    assert np.linalg.norm(out[0][0] - out[1][0]) <= 1e-7 * np.linalg.norm(out[0][0])
+    assert np.array_equal(out[0][1], out[1][1])   # thermal: the same kernel in both
+    assert np.abs(out[0][2][:6] - out[1][2][:6]).max() <= 2
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
