@@ -152,7 +152,7 @@ int maple_set_coarse_refresh(maple_ctx* ctx, int every);
  * which gather their shared neighbours once), 6 or 7 (2x2 nodes in x and z per
  * thread, three or two load cases) or 8 (as 5 with the z neighbours taken from
  * the neighbouring lanes of a wavefront, which holds one z-pencil; cells with
- * nz = 64, else 5 is used).  Default: 5.  The thermal operator always
+ * nz = 64, else 5 is used).  Default: 8.  The thermal operator always
  * uses the element-row kernel (variant 0, or 1 when 1 is set). */
 /* MAPLE_KERNEL_COARSE_OPERATOR: the elastic Galerkin stencil product of the
  * coarse levels (the coarse matvec of the V-cycle, EBE_GMG.py:886-984), variant

@@ -913,7 +913,9 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
       if ((rc = make_launch(c, L, 1024, false, &L.doblk, &L.oc))) return bail(rc);
       if ((rc = make_launch(c, L, 512, false, &L.dxblk, &L.pair, &L.cnt_pair))) return bail(rc);
       if ((rc = make_launch(c, L, 1024, false, &L.dqblk, &L.quad, &L.cnt_quad))) return bail(rc);
-      L.fine.var = 5;   // default level-0 elastic operator: 2x1 node blocks, three load cases per thread
+      // default level-0 elastic operator: 2x1 node blocks, three load cases per thread,
+      // z neighbours from the neighbouring lanes when every cell has nz = 64
+      L.fine.var = 8;
     } else {
       L.node.var = 2;   // default coarse elastic operator: node or row threads by launch size
     }

@@ -14,7 +14,7 @@ import torch
 from . import _lib as L
 from .cells import CellBatch
 
-KERNELS = {"fine_operator": (L.MAPLE_KERNEL_FINE_OPERATOR, (0, 1, 2, 3, 4, 5, 6, 7))}
+KERNELS = {"fine_operator": (L.MAPLE_KERNEL_FINE_OPERATOR, (0, 1, 2, 3, 4, 5, 6, 7, 8))}
 
 
 def _time(fn, reps: int) -> float:
