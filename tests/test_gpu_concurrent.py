@@ -1,0 +1,51 @@
+"""Concurrent groups of cells (maple_amd.concurrent.ConcurrentOC): every cell follows
+the trajectory it has in one lockstep batch, whatever group it runs in; errors on a
+group's thread reach the caller."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0) if torch.cuda.is_available() else None
+
+
+def _make(ncells, r=16):
+    import maple_amd
+    cb = maple_amd.CellBatch([r] * ncells, nlevels=3, device=DEV, precision="mixed", warm_start=True, islands=True)
+    drv = maple_amd.OCDriver(cb, "stiffness:0.5,thermal:0.5", 0.25, rtol=1e-8, maxit=3000)
+    return cb, drv
+
+
+def test_groups_follow_the_lockstep_trajectories():
+    from maple_amd.concurrent import ConcurrentOC
+    r, n, steps = 16, 4, 3
+    x0 = np.concatenate([np.clip(0.25 + np.random.default_rng(k).uniform(-0.1, 0.1, r ** 3), 1e-3, 1)
+                         for k in range(n)])
+    cb, drv = _make(n, r)
+    x = torch.from_numpy(x0).to(DEV)
+    for _ in range(steps):
+        x, info = drv.step(x)
+    ref = x.cpu().numpy()
+    cb.close()
+    groups = []
+    for g in range(2):
+        c, d = _make(2, r)
+        groups.append((c, d, torch.from_numpy(x0[g * 2 * r ** 3:(g + 1) * 2 * r ** 3].copy()).to(DEV)))
+    seen = []
+    res = ConcurrentOC(groups).run(steps, on_step=lambda g, k, info: seen.append((g, k)))
+    assert sorted(seen) == [(g, k) for g in range(2) for k in range(steps)]
+    got = np.concatenate([res[g][0].cpu().numpy() for g in range(2)])
+    assert np.abs(got - ref).max() <= 1e-9
+    for c, _, _ in groups:
+        c.close()
+
+
+def test_group_errors_reach_the_caller():
+    from maple_amd.concurrent import ConcurrentOC
+    import maple_amd
+    c, d = _make(1)
+    bad = torch.zeros(7, dtype=torch.float64, device=DEV)   # wrong size: the step raises
+    with pytest.raises(ValueError):
+        ConcurrentOC([(c, d, bad)]).run(1)
+    c.close()
