@@ -52,8 +52,8 @@ PEAK = {"fp64": 78.6, "mixed": 157.3}
 FLOP_PER_ELEM_ELASTIC = 1152      # 24 rows x 24 cols x 2 per element and load case
 # HBM bytes per 64^3 cell and launch of the level-0 elastic operator, from rocprofv3
 # FETCH_SIZE + WRITE_SIZE runs calibrated against a copy kernel
-# (profiles/fine_operator_64cube_hbm_counters.md): FP64 product 105.5 MB, FP32 Jacobi sweep 81.2 MB
-TRAFFIC_PER_CELL = {"fp64": 105.5e6, "mixed": 81.2e6}
+# (profiles/fine_operator_64cube_hbm_counters.md): FP64 product 93.5 MB, FP32 Jacobi sweep 81.5 MB
+TRAFFIC_PER_CELL = {"fp64": 93.5e6, "mixed": 81.5e6}
 
 
 def _x0(ncells):
@@ -156,7 +156,7 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
         avg_s = kms / count * 1e-3
         flops = work / count * FLOP_PER_ELEM_ELASTIC
         tname = "double" if precision == "fp64" else "float"
-        out["kernel"] = {"name": f"k_fine_nodeb<{tname},*,3,2,1> (level-0 elastic operator, node blocks, "
+        out["kernel"] = {"name": f"k_fine_nodeb64<{tname},*,3> (level-0 elastic operator, node blocks, "
                                  f"up to 6 load cases x {per} cells per launch)",
                          "launches": count, "avg_us": avg_s * 1e6, "flop_per_launch": flops,
                          "elem_cases_per_launch": work / count,

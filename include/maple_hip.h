@@ -157,7 +157,7 @@ int maple_set_coarse_refresh(maple_ctx* ctx, int every);
 /* MAPLE_KERNEL_COARSE_OPERATOR: the elastic Galerkin stencil product of the
  * coarse levels (the coarse matvec of the V-cycle, EBE_GMG.py:886-984), variant
  * 0 (one thread per node row, three times the waves), 1 (one thread per node,
- * each neighbour gathered once for the three rows) or 2 (1 for launches of 1024
+ * each neighbour gathered once for the three rows) or 2 (1 for launches of 256
  * or more blocks, 0 below; the default). */
 #define MAPLE_KERNEL_FINE_OPERATOR 0
 #define MAPLE_KERNEL_COARSE_OPERATOR 1

@@ -170,9 +170,10 @@ template <int ND, typename T>
 void stencil_apply(const Launch& L, int mode, int ncase, const T* S, const T* u, const T* f, const T* dinv,
                    T omega, T* out, double* /*partial*/, hipStream_t s) {
   (void)ncase;   // always all load cases of the physics (6 elastic, 3 thermal)
-  // variant 2 (default): the node form once the launch has 1024 or more blocks (the
-  // batch's level 1; it shrinks as cells converge), the row form below that
-  if (ND == 3 && (L.var == 1 || (L.var == 2 && L.nblocks >= 1024))) {
+  // variant 2 (default): the node form once the launch has 256 or more blocks (level 1
+  // of two or more 64^3 cells; it shrinks as cells converge), the row form below that
+  // (one cell's 32^3 level and every coarser one)
+  if (ND == 3 && (L.var == 1 || (L.var == 2 && L.nblocks >= 256))) {
     if (mode == 0) k_stencil_apply_node<T, 0><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, S, u, f, dinv, omega, out, L.done, L.ndc);
     else if (mode == 2) k_stencil_apply_node<T, 2><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, S, u, f, dinv, omega, out, L.done, L.ndc);
     else k_stencil_apply_node<T, 3><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, S, u, f, dinv, omega, out, L.done, L.ndc);
