@@ -15,7 +15,9 @@ Forms (one JSON line; ``value`` is the headline form):
                launches per OC step, its own stream and host thread),
                FP64 CG around an FP32 multigrid preconditioner with the FP64
                island correction (small solid islands solved exactly beside the
-               V-cycle), each solve starting from the best of zero, the previous
+               V-cycle) and its coarsest dense inverse rebuilt every third
+               density (an earlier density's SPD inverse in between), each
+               solve starting from the best of zero, the previous
                iterate's fields and those fields plus the change the learned
                network predicts between the two densities.
                value = cells x iterations / s.
@@ -64,8 +66,8 @@ def _x0(ncells):
 
 
 def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", warm: bool = False,
-             learned: bool = False, block: bool = False, islands: bool = False, groups: int = 1, maxit: int = 5000,
-             profile_kernel: bool = True):
+             learned: bool = False, block: bool = False, islands: bool = False, groups: int = 1, refresh: int = 1,
+             maxit: int = 5000, profile_kernel: bool = True):
     """groups > 1: the cells advance in that many concurrent groups (maple_amd.concurrent),
     each its own batch, stream and host thread; every cell's trajectory is unchanged."""
     import numpy as np
@@ -82,6 +84,7 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
         for g in range(groups):
             cells = maple_amd.CellBatch([R] * per, nlevels=NLEVELS, device=dev, warm_start=warm,
                                         precision=precision, block_cg=block, islands=islands)
+            cells.set_coarse_refresh(refresh)
             start = None
             if learned:
                 from maple_amd import learned as ml
@@ -177,6 +180,7 @@ def run_config4(steps: int, precision: str = "mixed", maxit: int = 5000):
     cells = maple_amd.CellBatch([c["r"] for c in spec], nlevels=NLEVELS, device=dev, precision=precision,
                                 warm_start=True)
     cells.set_islands(True)
+    cells.set_coarse_refresh(3)
     drv = maple_amd.OCDriver(cells, [c["weights"] for c in spec], [c["volfrac"] for c in spec], move=0.2, xmin=1e-3,
                              filter_k=3, rtol=1e-6, maxit=maxit)
     x = torch.from_numpy(np.concatenate([c["x0"] for c in spec])).to(dev)
@@ -210,10 +214,12 @@ def cpu_baseline(n_iter: int):
 
 FORMS = {
     "plain": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False, islands=False),
-    "headline": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True, groups=4),
-    "headline-lockstep": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True),
+    "headline": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True, groups=4,
+                     refresh=3),
+    "headline-lockstep": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True,
+                              refresh=3),
     "headline-block": dict(ncells=16, precision="mixed", warm=True, learned=True, block=True, islands=True,
-                           groups=4),
+                           groups=4, refresh=3),
     "fp64-warm": dict(ncells=1, precision="fp64", warm=True, learned=False, block=False, islands=False),
     "fp64-islands": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False, islands=True),
     "fp64-block": dict(ncells=1, precision="fp64", warm=False, learned=False, block=True, islands=False),
@@ -274,6 +280,7 @@ def main():
                    "form": head, "cells": spec["ncells"], "precision": spec["precision"],
                    "warm_start": spec["warm"], "learned_start": spec["learned"], "block_cg": spec["block"],
                    "island_correction": spec["islands"], "groups": spec.get("groups", 1),
+                   "coarsest_inverse_every": spec.get("refresh", 1),
                    "resolution": R,
                    "levels": NLEVELS, "rtol": 1e-6},
         "ch_rel_err_vs_fp64_tight_solve": g["ch_rel_err"],

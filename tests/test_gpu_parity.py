@@ -114,6 +114,26 @@ def test_lane_rotation_variant_64(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lagged_coarsest_inverse(precision):
+    """maple_set_coarse_refresh(3): the second and third densities in a row are solved
+    with the first one's coarsest inverse (still SPD), and each still meets the FP64
+    bar against the oracle."""
+    import maple_amd
+    r, nlev = 16, 3
+    cb = maple_amd.CellBatch([r], nlevels=nlev, device=DEV, precision=precision, islands=True)
+    cb.set_coarse_refresh(3)
+    cs = ohomog.CellSolver((r, r, r), nlevels=3, rtol=1e-12)
+    for k in range(3):
+        rho = _rho(r, 30 + k)
+        CH, kap, it = cb.homogenize(_t(rho), rtol=1e-10, maxit=3000)
+        CHo, _, _ = cs.elastic(rho)
+        ko, _, _ = cs.thermal(rho)
+        assert np.linalg.norm(CH[0].cpu().numpy() - CHo) <= 1e-9 * np.linalg.norm(CHo)
+        assert np.linalg.norm(kap[0].cpu().numpy() - ko) <= 1e-9 * np.linalg.norm(ko)
+    cb.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_coarse_operator_variant(precision):
     """The node-per-thread coarse stencil product (all rows of a node in one thread)
     inside the whole solve against the oracle, and against the row-per-thread form."""
