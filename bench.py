@@ -85,6 +85,10 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
             cells = maple_amd.CellBatch([R] * per, nlevels=NLEVELS, device=dev, warm_start=warm,
                                         precision=precision, block_cg=block, islands=islands)
             cells.set_coarse_refresh(refresh)
+            if groups > 1:
+                # the groups already overlap on the device: each solves its two physics in turn
+                # on its own stream (4 hardware queues per process)
+                cells.set_concurrency(False)
             start = None
             if learned:
                 from maple_amd import learned as ml
