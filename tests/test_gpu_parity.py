@@ -31,7 +31,7 @@ def _rho(r, seed, contrast=1e-9):
     return contrast + (1 - contrast) * x ** 3
 
 
-@pytest.mark.parametrize("r,nlev", [(8, 3), (16, 3), (32, 3), (72, 4)])
+@pytest.mark.parametrize("r,nlev", [(8, 3), (16, 3), (32, 3), (72, 4), (64, 4)])
 @pytest.mark.parametrize("physics", ["elastic", "thermal"])
 def test_operator_and_rhs(r, nlev, physics):
     print(f"[operator] r={r} {physics}", flush=True)
