@@ -11,8 +11,8 @@ It restates, in numpy/scipy FP64, the algorithm of the reference
 * ``fe``        element matrices            (_utils.py:39-196)
 * ``grid``      periodic hex8 grid, RHS      (main_IHD_v3.py:162-215, _utils.py:453-507)
 * ``mg``        converged solve: Galerkin multigrid-preconditioned CG
-                (EBE_GMG.py:202-430, 470-647; solver form per the
-                north-star spec: weighted Jacobi, dense coarsest solve)
+                (EBE_GMG.py:202-430, 470-647; converged PCG with a
+                weighted-Jacobi V-cycle and a dense coarsest solve)
 * ``homog``     energy-form C_H / kappa_H and closed-form sensitivities
                 (model.py:33-95; main_IHD_v3.py:688 at convergence)
 * ``objective`` stiffness / shear / thermal terms (objective.py:11-251)

@@ -8,7 +8,7 @@ Galerkin projection (``sparse_coarsen_ke``, EBE_GMG.py:226-337): a coarse
 element matrix is sum_c P_c^T K_child(c) P_c over its 8 children with the
 trilinear local interpolation (EBE_GMG.py:202-224).  The grid transfer is the
 periodic trilinear prolongation with weights prod_d (1 - |rem_d/2 - off_d|)
-(EBE_GMG.py:531-647).  The solver form is the north-star one (symmetric
+(EBE_GMG.py:531-647).  The solver form is a converged PCG (symmetric
 V-cycle with weighted Jacobi, exact coarsest solve with the translation
 constraint, CG with rtol on ||r||/||f||); the reference itself runs a single
 unconverged V-cycle, which is not a parity target.

@@ -1,5 +1,5 @@
 """GPU parity of the mixed-precision form (FP32 multigrid preconditioner inside
-FP64 CG) against the FP64 oracle.  Bar (north star): C_H and kappa_H within
+FP64 CG) against the FP64 oracle.  Bar: C_H and kappa_H within
 1e-5 Frobenius-relative at rtol 1e-6; sensitivities within 1e-5 of the largest."""
 import os
 
