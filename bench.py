@@ -249,7 +249,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X (cells shard by process "
+        print("bench.py measures one GPU: the solver runs on a single MI355X (cells shard by process "
               "per GPU with no collectives); --gpus above 1 is not supported.", file=sys.stderr)
         sys.exit(2)
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
