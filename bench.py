@@ -5,9 +5,13 @@ filter k=3 on dJ/dx, 6 elastic unit-strain + 3 thermal unit-gradient cell
 problems, 4-level multigrid-preconditioned CG to rtol 1e-6, J = 0.5 stiffness +
 0.5 kappa, volume fraction 0.25 on mean(x), move 0.2, x0 = 0.25 + U(-0.05, 0.05)
 (numpy default_rng(seed), cell k has seed k).  One OC iteration of a cell = both
-homogenizations, sensitivities, filter and OC update.  Every timed run is a
-whole 100-iteration trajectory from x0, including the hard middle where small
-disconnected islands slow the cell solves down.
+homogenizations, sensitivities, filter and OC update.
+
+A "step" is one OC iteration.  Every form's timed region is one whole
+100-iteration trajectory from x0 (OC iterations 1-100, including the hard middle
+where islands and thin members slow the cell solves down), whatever --steps asks
+for: ``steps`` reports the 100 iterations timed and ``steps_requested`` the flag.
+--warmup W runs W untimed OC iterations on separate contexts first.
 
 Forms (one JSON line; ``value`` is the headline form):
 * ``plain``    one cell, FP64, zero start every solve: the reference's method on the GPU.
@@ -21,18 +25,33 @@ Forms (one JSON line; ``value`` is the headline form):
                iterate's fields and those fields plus the change the learned
                network predicts between the two densities.
                value = cells x iterations / s.
+* ``headline-lockstep`` the same 16 cells as one batch (its level-0 operator
+               launches do not overlap another group's, so their HIP-event spans
+               are their durations: the roofline is read there).
 * optional (--forms): single-cell and batched ablations (``fp64-warm``,
   ``fp64-islands``, ``fp64-block``, ``mixed-warm``, ``mixed-warm-block``,
   ``mixed-warm-learned``, ``batched-mixed-warm``, ``batched-mixed-warm-islands``,
-  ``batched-mixed-warm-block``, ``headline-block`` with block CG over the load
-  cases, ``headline-lockstep`` with all 16 cells in one group); --config4 adds
-  the 64-cell mixed-resolution batch (64^3-equivalent iterations / s).
+  ``batched-mixed-warm-block``, ``headline-block``).
 
-Accuracy: the last iterate's C_H and kappa of the headline form (every cell)
-against a tight (rtol 1e-11) FP64 solve of the same densities.  ``roofline`` is
-measured live for the dominant kernel (the level-0 elastic operator of the
-headline form's preconditioner) with HIP events on the stream it runs on.
-``cpu_baseline`` times the FP64 CPU oracle on one OC iteration of the same cell.
+Beside the headline (each can be switched off):
+* ``config4``: the 64-cell mixed-resolution batch (16 x 32^3, 32 x 48^3, 16 x 64^3,
+  per-cell seeds, simplex weights and volume fractions), 100 OC iterations, cell
+  iterations / s and 64^3-equivalent iterations / s; solves that stop early are
+  counted, not fatal.
+* ``config3``: one 64^3 cell in the mixed form, J = 0.4 E + 0.3 G + 0.3 kappa, 100
+  OC iterations; at every iterate C_H, kappa_H and dJ/drho against a tight
+  (rtol 1e-10) FP64 solve of the same density (the FP64 path is pinned to the
+  FP64 oracle by the tests), max and last error reported.
+* ``config1_gpu`` and ``cpu_baseline``: the 32^3 sphere-void cell, bulk modulus,
+  3 levels, FP64, the plain method; the GPU runs 50 OC iterations, the CPU oracle
+  (FP64 numpy/scipy restatement, on the host cores) runs 2 warm-up iterations and
+  then --cpu-iters timed ones (median and total reported, with the CPU model);
+  the GPU's rate over the same iterations is reported beside it.
+
+Accuracy: the last iterate's C_H and kappa of every form against a tight
+(rtol 1e-10) FP64 solve of the same densities.  ``roofline`` is measured live
+for the dominant kernel (the level-0 elastic operator of the headline form's
+preconditioner) with HIP events on the stream it runs on.
 """
 from __future__ import annotations
 
@@ -50,12 +69,19 @@ METRIC = "OC iters/sec at 64^3 (mixed+batched+learned vs plain FP64 GPU) at C_H 
 R = 64
 NLEVELS = 4
 OBJ = "stiffness:0.5,thermal:0.5"
-PEAK = {"fp64": 78.6, "mixed": 157.3}
+# Peaks of MI355X for the dominant kernel's arithmetic (TFLOP/s).  The level-0 operator
+# runs on the vector ALU without packed FP32 (the build disables it): FP64 and
+# non-packed FP32 FMA both issue at 78.6; packed FP32 and FP32 MFMA reach 157.3.
+PEAK = {"fp64": 78.6, "mixed": 78.6}
+PEAK_PACKED_OR_MFMA = {"fp64": 78.6, "mixed": 157.3}
 FLOP_PER_ELEM_ELASTIC = 1152      # 24 rows x 24 cols x 2 per element and load case
-# HBM bytes per 64^3 cell and launch of the level-0 elastic operator, from rocprofv3
-# FETCH_SIZE + WRITE_SIZE runs calibrated against a copy kernel
-# (profiles/fine_operator_64cube_hbm_counters.md): FP64 product 93.5 MB, FP32 Jacobi sweep 81.5 MB
-TRAFFIC_PER_CELL = {"fp64": 93.5e6, "mixed": 81.5e6}
+TRAJ = 100                        # OC iterations of every timed trajectory
+# HBM bytes per element and load case of one level-0 elastic operator launch, from
+# the rocprofv3 FETCH_SIZE and WRITE_SIZE runs of profiles/fine_operator_64cube_hbm_counters.md
+# (copy-calibrated): FP64 product 93.5 MB, FP32 Jacobi sweep 81.5 MB per 64^3 cell and 6 cases
+TRAFFIC_PER_ELEM_CASE = {"fp64": 93.5e6 / (6 * 64 ** 3), "mixed": 81.5e6 / (6 * 64 ** 3)}
+TRAFFIC_SOURCE = "derived from profiles/fine_operator_64cube_hbm_counters.md (FETCH_SIZE + WRITE_SIZE, " \
+                 "copy-calibrated) x the element-cases of the launches timed here"
 
 
 def _x0(ncells):
@@ -134,7 +160,7 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
     CHl = np.concatenate([r[1]["CH"] for r in res])
     kl = np.concatenate([r[1]["kappa"] for r in res])
     ref = maple_amd.CellBatch([R] * ncells, nlevels=NLEVELS, device=dev)
-    CHt, kt, _ = ref.homogenize(rho, ("elastic", "thermal"), rtol=1e-11, maxit=4 * maxit, strict=False)
+    CHt, kt, _ = ref.homogenize(rho, ("elastic", "thermal"), rtol=1e-10, maxit=4 * maxit, strict=False)
     CH6 = torch.from_numpy(CHl).to(dev)
     k6 = torch.from_numpy(kl).to(dev)
     err = 0.0
@@ -168,12 +194,13 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
                          "launches": count, "avg_us": avg_s * 1e6, "flop_per_launch": flops,
                          "elem_cases_per_launch": work / count,
                          "achieved_tflops": flops / avg_s / 1e12, "peak": PEAK[precision],
-                         "traffic": work / count / (6 * R ** 3) * TRAFFIC_PER_CELL[precision]}
+                         "traffic": work / count * TRAFFIC_PER_ELEM_CASE[precision]}
     return out
 
 
-def run_config4(steps: int, precision: str = "mixed", maxit: int = 5000):
-    """The 64-cell mixed-resolution batch (16 x 32^3, 32 x 48^3, 16 x 64^3)."""
+def run_config4(steps: int, precision: str = "mixed", maxit: int = 3000):
+    """The 64-cell mixed-resolution batch (16 x 32^3, 32 x 48^3, 16 x 64^3) as one batch.
+    Solves that stop before rtol are counted (strict=False), not fatal."""
     import numpy as np
     import torch
 
@@ -182,38 +209,132 @@ def run_config4(steps: int, precision: str = "mixed", maxit: int = 5000):
     dev = torch.device("cuda", 0)
     spec = maple_amd.batched_workload()
     cells = maple_amd.CellBatch([c["r"] for c in spec], nlevels=NLEVELS, device=dev, precision=precision,
-                                warm_start=True)
-    cells.set_islands(True)
+                                warm_start=True, islands=True)
     cells.set_coarse_refresh(3)
     drv = maple_amd.OCDriver(cells, [c["weights"] for c in spec], [c["volfrac"] for c in spec], move=0.2, xmin=1e-3,
-                             filter_k=3, rtol=1e-6, maxit=maxit)
+                             filter_k=3, rtol=1e-6, maxit=maxit, physics=("elastic", "thermal"), strict=False)
     x = torch.from_numpy(np.concatenate([c["x0"] for c in spec])).to(dev)
+    unconv, worst = 0, 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         x, info = drv.step(x)
+        unconv += info["unconverged"]
+        worst = max(worst, int(info["iters"].max()))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     elems = sum(c["r"] ** 3 for c in spec)
     cells.close()
     return {"steps": steps, "ms_per_step": dt * 1e3 / steps, "cells": len(spec), "elements": elems,
-            "cell_oc_iters_per_s": len(spec) * steps / dt, "equiv_64cube_oc_iters_per_s": elems / R ** 3 * steps / dt}
+            "cell_oc_iters_per_s": len(spec) * steps / dt, "equiv_64cube_oc_iters_per_s": elems / R ** 3 * steps / dt,
+            "unconverged_solves": unconv, "max_cg_iters": worst,
+            "form": "one batch, mixed precision, warm start, island correction, coarsest inverse every 3rd density"}
 
 
-def cpu_baseline(n_iter: int):
-    from oracle import driver
+def run_config3(steps: int):
+    """One 64^3 cell, mixed form, tri-objective; per-iterate error against a tight FP64 solve."""
+    import numpy as np
+    import torch
+
+    import maple_amd
+
+    dev = torch.device("cuda", 0)
+    w = {"stiffness": 0.4, "shear": 0.3, "thermal": 0.3}
+    cm = maple_amd.CellBatch([R], nlevels=NLEVELS, device=dev, precision="mixed", warm_start=True)
+    cf = maple_amd.CellBatch([R], nlevels=NLEVELS, device=dev, warm_start=True)
+    drv = maple_amd.OCDriver(cm, w, 0.25, move=0.2, xmin=1e-3, filter_k=3, rtol=1e-6, maxit=5000)
+    x = torch.from_numpy(maple_amd.x0_uniform_noise(R, seed=0)).to(dev)
+    eC, ek, ed = [], [], []
+    t_mixed = 0.0
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, info = drv.step(x)
+        torch.cuda.synchronize()
+        t_mixed += time.perf_counter() - t0
+        CHf, kf, _ = cf.homogenize(info["rho"], rtol=1e-10, maxit=20000, strict=False)
+        df = cf.sensitivity(torch.from_numpy(info["G6"]).to(dev), torch.from_numpy(info["G3"]).to(dev))
+        CHf, kf = CHf[0].cpu().numpy(), kf[0].cpu().numpy()
+        eC.append(float(np.linalg.norm(info["CH"][0] - CHf) / np.linalg.norm(CHf)))
+        ek.append(float(np.linalg.norm(info["kappa"][0] - kf) / np.linalg.norm(kf)))
+        ed.append(float((info["dJdrho"] - df).abs().max() / df.abs().max()))
+    cm.close()
+    cf.close()
+    return {"steps": steps, "oc_iters_per_s_mixed": steps / t_mixed,
+            "ch_rel_err_max": max(eC), "kappa_rel_err_max": max(ek), "dJdrho_rel_err_max": max(ed),
+            "ch_rel_err_last": eC[-1], "kappa_rel_err_last": ek[-1], "dJdrho_rel_err_last": ed[-1],
+            "per_iteration": {"ch": [float(f"{v:.3g}") for v in eC], "kappa": [float(f"{v:.3g}") for v in ek],
+                              "dJdrho": [float(f"{v:.3g}") for v in ed]},
+            "reference_solve": "library FP64 path at rtol 1e-10 on the same density (pinned to the FP64 oracle "
+                               "by tests/test_gpu_configs.py at 64^3)"}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_config1_gpu(steps: int = 50, window=None):
+    """The 32^3 sphere-void cell, bulk modulus, 3 levels, FP64 plain method on the GPU.
+    window = (first, last) OC iterations (1-based) whose rate is also reported."""
+    import numpy as np
+    import torch
+
+    import maple_amd
+
+    dev = torch.device("cuda", 0)
+    r = 32
+    cells = maple_amd.CellBatch([r], nlevels=3, device=dev)
+    drv = maple_amd.OCDriver(cells, "bulk:1.0", 0.25, move=0.2, xmin=1e-3, filter_k=3, rtol=1e-6, maxit=5000)
+    x = torch.from_numpy(maple_amd.x0_sphere_void(r, seed=0)).to(dev)
+    times = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, info = drv.step(x)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    cells.close()
+    out = {"oc_iters_per_s": steps / sum(times), "steps": steps, "median_s": float(np.median(times)),
+           "J_last": float(info["J"][0])}
+    if window:
+        a, b = window
+        out["window"] = f"OC iterations {a}-{b}"
+        out["window_oc_iters_per_s"] = (b - a + 1) / sum(times[a - 1:b])
+    return out
+
+
+def cpu_baseline(n_iter: int, warmup: int = 2):
+    """Config 1 on the host cores: the FP64 oracle (numpy/scipy restatement of the plain
+    method), warmup untimed OC iterations then n_iter timed ones."""
+    import numpy as np
 
     cores = len(os.sched_getaffinity(0))
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    x0 = driver.x0_uniform_noise(R, seed=0)
+    try:
+        import torch
+        torch.set_num_threads(cores)
+    except Exception:
+        pass
+    from oracle import driver
+
+    x0 = driver.x0_sphere_void(32, seed=0)
     t0 = time.perf_counter()
-    x, hist, times = driver.run(R, x0, {"stiffness": 0.5, "thermal": 0.5}, n_iter, volfrac=0.25, nlevels=NLEVELS,
-                                rtol=1e-6, move=0.2, xmin=1e-3)
+    _, _, times = driver.run(32, x0, {"bulk": 1.0}, n_iter, volfrac=0.25, nlevels=3, rtol=1e-6, move=0.2, xmin=1e-3,
+                             warmup=warmup)
     total = time.perf_counter() - t0
-    per = sum(times) / len(times)
-    return {"value": 1.0 / per, "unit": "OC iters/sec", "cores": cores, "kind": "port",
-            "sample": f"{n_iter} OC iteration(s) of the same 64^3 workload from x0, FP64 numpy/scipy oracle "
-                      f"(multithreaded BLAS/sparse kernels), {total:.1f} s wall"}
+    return {"value": n_iter / sum(times), "unit": "OC iters/sec", "cores": cores, "kind": "port",
+            "cpu_model": _cpu_model(), "median_s_per_iter": float(np.median(times)),
+            "total_timed_s": float(sum(times)), "wall_s": total,
+            "sample": f"config 1 (32^3 sphere-void cell, bulk modulus, 3-level MG-PCG rtol 1e-6, FP64): "
+                      f"{warmup} warm-up + {n_iter} timed OC iterations (iterations {warmup + 1}-{warmup + n_iter}) "
+                      f"of the FP64 numpy/scipy oracle on {cores} host cores"}
 
 
 FORMS = {
@@ -241,11 +362,14 @@ FORMS = {
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=TRAJ,
+                    help="kept for the harness; every form times one whole 100-iteration trajectory")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed OC iterations on separate contexts first")
     ap.add_argument("--forms", default="plain,headline,headline-lockstep", help="comma list of " + ", ".join(FORMS))
-    ap.add_argument("--config4", action="store_true", help="also run the 64-cell mixed-resolution batch")
-    ap.add_argument("--cpu-iters", type=int, default=1)
+    ap.add_argument("--no-config4", action="store_true", help="skip the 64-cell mixed-resolution batch")
+    ap.add_argument("--no-config3", action="store_true", help="skip the per-iterate mixed-form error report")
+    ap.add_argument("--no-config1", action="store_true", help="skip the 32^3 config-1 GPU run")
+    ap.add_argument("--cpu-iters", type=int, default=8, help="timed CPU-oracle OC iterations after 2 warm-up ones")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -258,9 +382,9 @@ def main():
             raise SystemExit(f"unknown form {f}")
     res = {}
     for f in forms:
-        res[f] = run_form(args.steps, args.warmup, **FORMS[f])
-        print(f"# {f}: {res[f]['value']:.3f} 64^3 OC it/s, C_H/kappa rel err {res[f]['ch_rel_err']:.2e}, "
-              f"elastic CG mean {res[f]['iters_el_mean']:.1f}", file=sys.stderr, flush=True)
+        res[f] = run_form(TRAJ, args.warmup, **FORMS[f])
+        print(f"# {f}: {res[f]['value']:.3f} 64^3 OC it/s over OC iterations 1-{TRAJ}, C_H/kappa rel err "
+              f"{res[f]['ch_rel_err']:.2e}, elastic CG mean {res[f]['iters_el_mean']:.1f}", file=sys.stderr, flush=True)
     ok = [f for f in forms if res[f]["ch_rel_err"] <= 1e-5]
     head = "headline" if "headline" in ok else (max(ok, key=lambda f: res[f]["value"]) if ok else forms[0])
     g = res[head]
@@ -270,7 +394,8 @@ def main():
         "value": g["value"],
         "unit": "OC iters/sec",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": TRAJ,
+        "steps_requested": args.steps,
         "warmup": args.warmup,
         "ms_per_step": g["ms_per_step"],
         "higher_is_better": True,
@@ -278,14 +403,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if spec["precision"] == "fp64" else "f64+f32 (+bf16 network)" if spec["learned"] else "f64+f32",
         "data": "synthetic",
-        "config": {"workload": f"{spec['ncells']} x 64^3 periodic cell(s), 100 OC iterations each from "
-                               "x0=0.25+U(-0.05,0.05) (seed = cell index), J=0.5E+0.5kappa, vf 0.25, move 0.2, "
-                               "filter k=3, 4-level MG-PCG rtol 1e-6",
+        "config": {"workload": f"{spec['ncells']} x 64^3 periodic cell(s), OC iterations 1-{TRAJ} (the whole "
+                               f"{TRAJ}-iteration trajectory) of each from x0=0.25+U(-0.05,0.05) (seed = cell index), "
+                               "J=0.5E+0.5kappa, vf 0.25, move 0.2, filter k=3, 4-level MG-PCG rtol 1e-6",
                    "form": head, "cells": spec["ncells"], "precision": spec["precision"],
                    "warm_start": spec["warm"], "learned_start": spec["learned"], "block_cg": spec["block"],
                    "island_correction": spec["islands"], "groups": spec.get("groups", 1),
                    "coarsest_inverse_every": spec.get("refresh", 1),
-                   "resolution": R,
+                   "resolution": R, "oc_iterations_timed": TRAJ,
                    "levels": NLEVELS, "rtol": 1e-6},
         "ch_rel_err_vs_fp64_tight_solve": g["ch_rel_err"],
         "forms": {f: {k: res[f][k] for k in ("value", "ms_per_step", "cells", "groups", "ch_rel_err", "iters_el_mean",
@@ -300,18 +425,37 @@ def main():
     rl_form = "headline-lockstep" if (head == "headline" and "headline-lockstep" in res) else head
     k = res[rl_form].get("kernel")
     if k:
-        line["roofline"] = {"bound": "mfma", "achieved": k["achieved_tflops"], "peak": k["peak"],
-                            "unit": "TFLOP/s", "frac": k["achieved_tflops"] / k["peak"], "traffic": k["traffic"],
-                            "traffic_unit": "bytes per launch (rocprofv3 PMC, copy-calibrated)",
-                            "kernel": k["name"], "avg_us": k["avg_us"], "launches": k["launches"],
-                            "algorithmic_flop_per_launch": k["flop_per_launch"],
-                            "measured_on_form": rl_form,
-                            "roof": "vector FMA and MFMA share the MI355X peak for this precision "
-                                    "(FP64 78.6, FP32 157.3 TFLOP/s)"}
-    if args.config4:
-        line["config4"] = run_config4(args.steps)
+        prec = FORMS[rl_form]["precision"]
+        line["roofline"] = {
+            "bound": "valu-fp64" if prec == "fp64" else "valu-fp32",
+            "achieved": k["achieved_tflops"], "peak": k["peak"], "unit": "TFLOP/s",
+            "frac": k["achieved_tflops"] / k["peak"],
+            "frac_of_packed_or_mfma_peak": k["achieved_tflops"] / PEAK_PACKED_OR_MFMA[prec],
+            "traffic": k["traffic"], "traffic_unit": "HBM bytes per launch, " + TRAFFIC_SOURCE,
+            "kernel": k["name"], "avg_us": k["avg_us"], "launches": k["launches"],
+            "elem_cases_per_launch": k["elem_cases_per_launch"],
+            "algorithmic_flop_per_launch": k["flop_per_launch"],
+            "flop_per_elem_case": FLOP_PER_ELEM_ELASTIC, "measured_on_form": rl_form,
+            "roof": "the kernel issues non-packed vector FMA (no MFMA, packed FP32 disabled in the build): "
+                    "FP64 and non-packed FP32 issue at 78.6 TFLOP/s on MI355X; packed FP32 / FP32 MFMA would "
+                    "reach 157.3"}
+    if not args.no_config4:
+        line["config4"] = run_config4(TRAJ)
+        print(f"# config4: {line['config4']['equiv_64cube_oc_iters_per_s']:.1f} 64^3-equivalent OC it/s",
+              file=sys.stderr, flush=True)
+    if not args.no_config3:
+        line["config3"] = run_config3(TRAJ)
+        print(f"# config3: max C_H err {line['config3']['ch_rel_err_max']:.2e}, dJ/drho "
+              f"{line['config3']['dJdrho_rel_err_max']:.2e}", file=sys.stderr, flush=True)
+    window = (3, 2 + args.cpu_iters)
+    if not args.no_config1:
+        line["config1_gpu"] = run_config1_gpu(50, window)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_iters)
+        if "config1_gpu" in line:
+            line["cpu_baseline"]["gpu_same_iterations_oc_iters_per_s"] = line["config1_gpu"]["window_oc_iters_per_s"]
+            line["cpu_baseline"]["gpu_over_cpu"] = line["config1_gpu"]["window_oc_iters_per_s"] / \
+                line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
 
 

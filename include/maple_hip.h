@@ -32,6 +32,8 @@ extern "C" {
 #define MAPLE_ENOMEM (-3)      /* device allocation failed */
 #define MAPLE_ENOCONV (-4)     /* CG hit maxit before rtol (results still written) */
 #define MAPLE_ENONFINITE (-5)  /* NaN / Inf in the input or the iteration */
+#define MAPLE_EBREAKDOWN (-6)  /* CG met p^T K p <= 0 on a live load case (operator or preconditioner
+                                  not SPD there): that case stopped unconverged; results still written */
 
 #define MAPLE_ELASTIC 1
 #define MAPLE_THERMAL 2
@@ -152,15 +154,21 @@ int maple_set_coarse_refresh(maple_ctx* ctx, int every);
  * which gather their shared neighbours once), 6 or 7 (2x2 nodes in x and z per
  * thread, three or two load cases) or 8 (as 5 with the z neighbours taken from
  * the neighbouring lanes of a wavefront, which holds one z-pencil; cells with
- * nz = 64, else 5 is used).  Default: 8.  The thermal operator always
- * uses the element-row kernel (variant 0, or 1 when 1 is set). */
+ * nz = 64, else 5 is used).  Default: 8. */
 /* MAPLE_KERNEL_COARSE_OPERATOR: the elastic Galerkin stencil product of the
  * coarse levels (the coarse matvec of the V-cycle, EBE_GMG.py:886-984), variant
  * 0 (one thread per node row, three times the waves), 1 (one thread per node,
  * each neighbour gathered once for the three rows) or 2 (1 for launches of 256
  * or more blocks, 0 below; the default). */
+/* MAPLE_KERNEL_THERMAL_OPERATOR: the level-0 thermal operator
+ * (EBEOperatorScalar.matvec, EBE_GMG_scalar.py:292-306), variant 0 or 1 (element
+ * rows per pencil lane, as variants 0 and 1 above) or 2 (node form: all load cases
+ * per thread, z neighbours from the neighbouring lanes, the element matrix's four
+ * Hamming-distance entries instead of 64; cells with nz = 64 whose thermal element
+ * matrix is a Q1 Laplacian, else variant 0 runs).  Default: 2. */
 #define MAPLE_KERNEL_FINE_OPERATOR 0
 #define MAPLE_KERNEL_COARSE_OPERATOR 1
+#define MAPLE_KERNEL_THERMAL_OPERATOR 2
 int maple_set_variant(maple_ctx* ctx, int kernel, int variant);
 
 /* Homogenize: for every cell, solve the 6 elastic unit-strain (and/or 3 thermal
@@ -175,7 +183,8 @@ int maple_set_variant(maple_ctx* ctx, int kernel, int variant);
  * Replaces IHD.TO_homo_3d / TO_homo_1d (model.py:98-159) with
  * GMGSolver.solve / GMGSolverScalar.solve (EBE_GMG.py:758-885,
  * EBE_GMG_scalar.py:617-725) and homogenized_batch_3d / _thermal
- * (model.py:33-95).  Returns MAPLE_ENOCONV if any case hit maxit. */
+ * (model.py:33-95).  Returns MAPLE_ENOCONV if any case hit maxit, MAPLE_EBREAKDOWN if
+ * any case broke down (both still write C_H / kappa_H of the fields reached). */
 int maple_homogenize(maple_ctx* ctx, const double* rho_dev, int physics, double rtol, int maxit, double* CH_dev,
                      double* kappa_dev, int* iters_host, void* stream);
 
@@ -246,6 +255,55 @@ long long maple_num_nodes(const maple_ctx* ctx, int level);
  * the reference. */
 int maple_timing_enable(maple_ctx* ctx, int enable);
 int maple_timing_read(maple_ctx* ctx, int kind, long long* count, double* total_ms, long long* work);
+
+/* ---- multigrid internals ----------------------------------------------------
+ * The hierarchy of one density field, piece by piece, for checks against a
+ * dense Galerkin construction and for tools.  `prec` selects the FP64
+ * hierarchy (MAPLE_PREC_FP64, every context) or the FP32 preconditioner of a
+ * MAPLE_PREC_MIXED context; inputs and outputs are FP64 device arrays in the
+ * plane-major layout of the level, every load case of the physics at once.
+ * These calls clear the convergence flags of the last solve of that physics. */
+
+/* Build the hierarchy of rho (Jacobi diagonals, Galerkin stencils, coarsest
+ * inverse) without solving; the FP64 level 0 keeps reading rho_dev, which must
+ * stay valid while the level-0 operator is used.  Replaces the per-solve build
+ * of GMGSolver.solve (EBE_GMG.py:790-849). */
+int maple_mg_setup(maple_ctx* ctx, int physics, const double* rho_dev, int prec, void* stream);
+
+/* Assembled Galerkin operator P^T A P of level `level` >= 1 as the library's
+ * 27-point block stencil: S[(d*nd*nd + i*nd + j)*N + n] is entry (i, j) of the
+ * block coupling node n to node n + off(d), d = (dx+1)*9 + (dy+1)*3 + (dz+1)
+ * (27*nd*nd*N_level values).  The reference's sparse_coarsen_ke
+ * (EBE_GMG.py:226-337) builds the same operator as per-element matrices. */
+int maple_mg_stencil(maple_ctx* ctx, int physics, int prec, int level, double* S_dev, void* stream);
+
+/* Inverse smoother diagonal of level `level` < nlevels-1 (nd*N_level values):
+ * 1/diag(A) for MAPLE_SMOOTH_JACOBI (_compute_diag_inverse, EBE_GMG.py:375-393,
+ * zero -> 1), 1/(absolute row sum) for MAPLE_SMOOTH_L1_JACOBI. */
+int maple_mg_dinv(maple_ctx* ctx, int physics, int prec, int level, double* dinv_dev, void* stream);
+
+/* Operator of one level on every load case: mode 0 out = A u, 2 out = f - A u,
+ * 3 out = u + omega Dinv (f - A u) (one smoother sweep; smooth_jacobi,
+ * EBE_GMG.py:395-411).  Level 0 is the matrix-free voxel operator
+ * (EBEOperator.matvec, EBE_GMG.py:412-430), the others their stencils. */
+int maple_mg_apply(maple_ctx* ctx, int physics, int prec, int level, int mode, const double* u_dev,
+                   const double* f_dev, double* out_dev, void* stream);
+
+/* coarse = P^T fine: periodic full-weighting restriction from level-1 to
+ * `level` (ProlongStencil.restrict_PT, EBE_GMG.py:59-79). */
+int maple_mg_restrict(maple_ctx* ctx, int physics, int level, const double* fine_dev, double* coarse_dev,
+                      void* stream);
+
+/* z = M r: one application of the V-cycle preconditioner at level 0 (no island
+ * correction), as inside the CG (GMGSolver._v_cycle, EBE_GMG.py:886-984). */
+int maple_mg_vcycle(maple_ctx* ctx, int physics, int prec, const double* r_dev, double* z_dev, void* stream);
+
+/* The coarsest level's dense inverse (A + gamma Z Z^T)^-1 of one cell, Z the
+ * normalised rigid translations and gamma the mean diagonal of A, row-major
+ * (order x order doubles, order = maple_mg_coarsest_order).  The reference's
+ * coarsest level is smoothing only (EBE_GMG.py:918-929). */
+int maple_mg_coarsest_inverse(maple_ctx* ctx, int physics, int cell, double* Ainv_dev, void* stream);
+int maple_mg_coarsest_order(const maple_ctx* ctx, int physics, int cell);
 
 /* ---- learned warm start ---------------------------------------------------
  * A 3-level periodic 3-D U-Net (channels 16-32-64, 3x3x3 convolutions with

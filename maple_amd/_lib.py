@@ -18,6 +18,7 @@ MAPLE_EDEVICE = -2
 MAPLE_ENOMEM = -3
 MAPLE_ENOCONV = -4
 MAPLE_ENONFINITE = -5
+MAPLE_EBREAKDOWN = -6
 MAPLE_ELASTIC = 1
 MAPLE_THERMAL = 2
 MAPLE_PREC_FP64 = 0
@@ -26,6 +27,7 @@ MAPLE_SMOOTH_JACOBI = 0
 MAPLE_SMOOTH_L1_JACOBI = 1
 MAPLE_KERNEL_FINE_OPERATOR = 0
 MAPLE_KERNEL_COARSE_OPERATOR = 1
+MAPLE_KERNEL_THERMAL_OPERATOR = 2
 
 _P = C.c_void_p
 _D = C.c_double
@@ -68,6 +70,14 @@ SIGNATURES = {
     "maple_num_nodes": (C.c_longlong, [_P, _I]),
     "maple_timing_enable": (_I, [_P, _I]),
     "maple_timing_read": (_I, [_P, _I, C.POINTER(C.c_longlong), _DP, C.POINTER(C.c_longlong)]),
+    "maple_mg_setup": (_I, [_P, _I, _P, _I, _P]),
+    "maple_mg_stencil": (_I, [_P, _I, _I, _I, _P, _P]),
+    "maple_mg_dinv": (_I, [_P, _I, _I, _I, _P, _P]),
+    "maple_mg_apply": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P]),
+    "maple_mg_restrict": (_I, [_P, _I, _I, _P, _P, _P]),
+    "maple_mg_vcycle": (_I, [_P, _I, _I, _P, _P, _P]),
+    "maple_mg_coarsest_inverse": (_I, [_P, _I, _I, _P, _P]),
+    "maple_mg_coarsest_order": (_I, [_P, _I, _I]),
     "maple_unet_num_params": (C.c_longlong, []),
     "maple_unet_create": (_I, [_I, _I, _I, C.POINTER(_P)]),
     "maple_unet_destroy": (_I, [_P]),
@@ -90,6 +100,10 @@ class MapleError(RuntimeError):
 
 class NotConverged(MapleError):
     pass
+
+
+class Breakdown(NotConverged):
+    """CG met p^T K p <= 0 on a live load case (results of the fields reached are written)."""
 
 
 def load(path: str = LIB_PATH):
@@ -116,6 +130,8 @@ def check(rc: int):
         raise ValueError(f"maple_hip: {msg}")
     if rc == MAPLE_ENOCONV:
         raise NotConverged(rc, msg)
+    if rc == MAPLE_EBREAKDOWN:
+        raise Breakdown(rc, msg)
     raise MapleError(rc, msg)
 
 

@@ -70,6 +70,7 @@ class CellBatch:
         self.nelem = int(self.offsets[-1])
         self.generation = {"elastic": 0, "thermal": 0}
         self.last_iters = None
+        self.last_status = 0
         self.set_smoother(smoother)
         if omega is None:
             omega = 1.6 if smoother == "l1-jacobi" else 0.6
@@ -181,7 +182,8 @@ class CellBatch:
         self.last_iters = iters.reshape(self.ncells, 9)
         for p in physics:
             self.generation[p] += 1
-        if rc != L.MAPLE_OK and (strict or rc != L.MAPLE_ENOCONV):
+        self.last_status = rc
+        if rc != L.MAPLE_OK and (strict or rc not in (L.MAPLE_ENOCONV, L.MAPLE_EBREAKDOWN)):
             L.check(rc)
         return CH, kap, self.last_iters
 
@@ -278,6 +280,71 @@ class CellBatch:
         L.check(self.lib.maple_prolong(self._h, PHYS[physics], int(level), C.c_void_p(coarse.data_ptr()),
                                        C.c_void_p(fine.data_ptr()), self._stream()))
         return fine
+
+    # -- multigrid internals -------------------------------------------------
+    def _prec(self, precision: str) -> int:
+        return {"fp64": L.MAPLE_PREC_FP64, "mixed": L.MAPLE_PREC_MIXED}[precision]
+
+    def mg_setup(self, physics: str, rho: torch.Tensor, precision: str = "fp64"):
+        """Build the hierarchy of rho without solving (the FP64 level 0 keeps reading
+        rho, so the returned tensor must stay alive while level 0 is used)."""
+        rho = self._vec(rho, self.nelem, "rho")
+        L.check(self.lib.maple_mg_setup(self._h, PHYS[physics], C.c_void_p(rho.data_ptr()), self._prec(precision),
+                                        self._stream()))
+        self._mg_rho = rho
+        return rho
+
+    def mg_stencil(self, physics: str, level: int, precision: str = "fp64") -> torch.Tensor:
+        """(27, nd, nd, N_level): entry [d, i, j, n] couples node n (row i) to node
+        n + off(d) (column j), d = (dx+1)*9 + (dy+1)*3 + (dz+1)."""
+        nd = NDOF[physics]
+        out = self._empty(27, nd, nd, self.num_nodes(level))
+        L.check(self.lib.maple_mg_stencil(self._h, PHYS[physics], self._prec(precision), int(level),
+                                          C.c_void_p(out.data_ptr()), self._stream()))
+        return out
+
+    def mg_dinv(self, physics: str, level: int, precision: str = "fp64") -> torch.Tensor:
+        """Inverse smoother diagonal (nd, N_level)."""
+        out = self._empty(NDOF[physics], self.num_nodes(level))
+        L.check(self.lib.maple_mg_dinv(self._h, PHYS[physics], self._prec(precision), int(level),
+                                       C.c_void_p(out.data_ptr()), self._stream()))
+        return out
+
+    def mg_apply(self, physics: str, level: int, u: torch.Tensor, f: Optional[torch.Tensor] = None, mode: int = 0,
+                 precision: str = "fp64") -> torch.Tensor:
+        """mode 0: A u; 2: f - A u; 3: one smoother sweep u + omega Dinv (f - A u); fields
+        of shape (ncase, nd, N_level)."""
+        n = NCASE[physics] * NDOF[physics] * self.num_nodes(level)
+        u = self._vec(u, n, "u")
+        fv = self._vec(f, n, "f") if f is not None else None
+        out = self._empty(NCASE[physics], NDOF[physics], self.num_nodes(level))
+        L.check(self.lib.maple_mg_apply(self._h, PHYS[physics], self._prec(precision), int(level), int(mode),
+                                        C.c_void_p(u.data_ptr()), C.c_void_p(fv.data_ptr()) if fv is not None else None,
+                                        C.c_void_p(out.data_ptr()), self._stream()))
+        return out
+
+    def mg_restrict(self, physics: str, level: int, fine: torch.Tensor) -> torch.Tensor:
+        """P^T fine from level - 1 to level, (ncase, nd, N_level)."""
+        fine = self._vec(fine, NCASE[physics] * NDOF[physics] * self.num_nodes(level - 1), "fine")
+        out = self._empty(NCASE[physics], NDOF[physics], self.num_nodes(level))
+        L.check(self.lib.maple_mg_restrict(self._h, PHYS[physics], int(level), C.c_void_p(fine.data_ptr()),
+                                           C.c_void_p(out.data_ptr()), self._stream()))
+        return out
+
+    def mg_vcycle(self, physics: str, r: torch.Tensor, precision: str = "fp64") -> torch.Tensor:
+        """One V-cycle z = M r at level 0 (every load case)."""
+        r = self._vec(r, NCASE[physics] * NDOF[physics] * self.nelem, "r")
+        out = self._empty(NCASE[physics], NDOF[physics], self.nelem)
+        L.check(self.lib.maple_mg_vcycle(self._h, PHYS[physics], self._prec(precision), C.c_void_p(r.data_ptr()),
+                                         C.c_void_p(out.data_ptr()), self._stream()))
+        return out
+
+    def mg_coarsest_inverse(self, physics: str, cell: int = 0) -> torch.Tensor:
+        n = int(self.lib.maple_mg_coarsest_order(self._h, PHYS[physics], int(cell)))
+        out = self._empty(n, n)
+        L.check(self.lib.maple_mg_coarsest_inverse(self._h, PHYS[physics], int(cell), C.c_void_p(out.data_ptr()),
+                                                   self._stream()))
+        return out
 
     # -- timing -------------------------------------------------------------
     def timing(self, enable: bool):

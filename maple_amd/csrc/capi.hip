@@ -67,6 +67,8 @@ struct VC {
 struct Phys {
   int nd = 0, nc = 0;
   double *Ke = nullptr, *Fe = nullptr, *X0 = nullptr, *G = nullptr;   // per cell on device
+  double* lap = nullptr;   // thermal: Q1-Laplacian coefficients k[0..3] per cell (node-form operator)
+  bool laplacian = false;  // thermal: every cell's Ke is a Q1 Laplacian (k[Hamming distance])
   std::vector<double> hKe, hFe, hX0;                                   // per cell on host
   VC<double> v64;
   VC<float> v32;
@@ -79,8 +81,8 @@ struct Phys {
   double* Se = nullptr;
   CaseScalars* sc = nullptr;
   std::vector<CaseScalars> hsc;
-  int* done_d = nullptr;
-  int* done_h = nullptr;   // pinned, 2 slots
+  int* done_d = nullptr;   // ncells*nc convergence flags, then the solve's CG restart flag
+  int* done_h = nullptr;   // pinned, 2 slots of ncells*nc + 1
   std::vector<Launch> ln;  // per-level node launches that skip converged (cell, case) pairs
   Launch lf;               // level-0 operator launch, same
   Launch lx, lq;           // level-0 node-pair and node-quad operator launches, same
@@ -107,6 +109,7 @@ struct Phys {
   double* isl_inv = nullptr;  // island correction: inverse blocks of this physics, per island
   int* isl_valid = nullptr;
   long long setups = 0;    // hierarchy builds so far (drives the coarsest-inverse refresh)
+  bool built[2] = {false, false};   // a hierarchy exists: [0] FP64, [1] FP32
 };
 
 struct Timer {
@@ -119,6 +122,7 @@ struct Timer {
 struct maple_ctx {
   int device = 0, ncells = 0, nlevels = 0, prec = 0;
   bool all_nz64 = false;   // every cell has nz = 64 (level-0 operator variant 8 applies)
+  int thermal_var = 2;     // thermal level-0 operator: 0/1 element rows, 2 node form where it applies
   std::vector<int> dims;
   std::vector<Level> lv;
   Phys ph[2];
@@ -214,6 +218,30 @@ int upload_phys_constants(maple_ctx* c, Phys& P) {
     at[i] = 1e-28 * fn * E;
   }
   HIPCHK(hipMemcpy(P.atol2, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (P.nd == 1) {
+    // node-form thermal operator: Ke[a][b] = k[d(a, b)], d the Hamming distance of corners a, b
+    std::vector<double> lap((size_t)c->ncells * 4, 0.0);
+    bool ok = true;
+    for (int i = 0; i < c->ncells && ok; ++i) {
+      const double* K = &P.hKe[(size_t)i * 64];
+      double* k = &lap[(size_t)i * 4];
+      bool seen[4] = {false, false, false, false};
+      double scale = 0.0;
+      for (int e = 0; e < 64; ++e) scale = std::max(scale, std::fabs(K[e]));
+      for (int a = 0; a < 8 && ok; ++a)
+        for (int b = 0; b < 8 && ok; ++b) {
+          const int d = (hex_dx(a) != hex_dx(b)) + (hex_dy(a) != hex_dy(b)) + (hex_dz(a) != hex_dz(b));
+          if (!seen[d]) {
+            k[d] = K[a * 8 + b];
+            seen[d] = true;
+          } else if (std::fabs(K[a * 8 + b] - k[d]) > 1e-14 * scale) {
+            ok = false;
+          }
+        }
+    }
+    P.laplacian = ok;
+    HIPCHK(hipMemcpy(P.lap, lap.data(), lap.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   return 0;
 }
 
@@ -227,6 +255,7 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
   TRY(dalloc(c, &P.X0, (size_t)c->ncells * ne * nc));
   TRY(dalloc(c, &P.G, (size_t)c->ncells * 8 * ne * ne));
   TRY(dalloc(c, &P.atol2, (size_t)c->ncells));
+  TRY(dalloc(c, &P.lap, (size_t)c->ncells * 4));
   auto alloc_vc = [&](auto& V) -> int {
     using TT = typename std::remove_pointer<typename std::decay<decltype(V.S[0])>::type>::type;
     V.S.assign(L, nullptr);
@@ -291,7 +320,7 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
   HIPCHK(hipMemcpy(P.ddc, P.hdc.data(), P.hdc.size() * sizeof(DenseCell), hipMemcpyHostToDevice));
   TRY(dalloc(c, &P.Se, (size_t)nc * (nc + 1) / 2 * c->lv[0].Ntot));
   TRY(dalloc(c, &P.sc, (size_t)c->ncells * nc));
-  TRY(dalloc(c, &P.done_d, (size_t)c->ncells * nc));
+  TRY(dalloc(c, &P.done_d, (size_t)c->ncells * nc + 1));
   P.ln.resize(L);
   for (int l = 0; l < L; ++l) {
     P.ln[l] = c->lv[l].node;
@@ -329,7 +358,7 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
   P.nlive = c->ncells;
   P.hsc.resize((size_t)c->ncells * nc);
   void* hp = nullptr;
-  HIPCHK(hipHostMalloc(&hp, sizeof(int) * 2 * c->ncells * nc, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc(&hp, sizeof(int) * 2 * (c->ncells * nc + 1), hipHostMallocDefault));
   c->host_allocs.push_back(hp);
   P.done_h = (int*)hp;
   return 0;
@@ -362,9 +391,17 @@ void timed_end(maple_ctx* c, int kind, hipStream_t s) {
 // kernels, or the node launch of the node-block kernels (elastic, variants 2-3).
 // full: every cell and case (no convergence flags).
 Launch op_launch(maple_ctx* c, const Phys& P, bool full) {
+  if (P.nd == 1) {
+    // thermal: node form (nz = 64 cells, Q1-Laplacian element matrices) or element rows
+    int tv = c->thermal_var;
+    if (tv == 2 && !(c->all_nz64 && P.laplacian)) tv = 0;
+    Launch L = tv == 2 ? (full ? c->lv[0].pair : P.lx) : (full ? c->lv[0].fine : P.lf);
+    L.var = tv;
+    return L;
+  }
   int v = c->lv[0].fine.var;
   if (v == 8 && !c->all_nz64) v = 5;                     // the lane-rotation form needs nz = 64
-  const int var = P.nd == 3 ? v : (v == 1 ? 1 : 0);   // thermal: element rows only
+  const int var = v;
   Launch L = var == 8 ? (full ? c->lv[0].pair : P.lx)
            : var >= 6 ? (full ? c->lv[0].quad : P.lq)
            : var >= 4 ? (full ? c->lv[0].pair : P.lx)
@@ -390,6 +427,8 @@ void apply_fine64(maple_ctx* c, Phys& P, int mode, const double* rho, const doub
     fine_apply_node<double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
   else if (P.nd == 3)
     fine_apply<3, double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
+  else if (L.var == 2)
+    thermal_apply64<double>(L, mode, P.nc, P.lap, rho, u, f, dinv, c->omega, out, partial, s);
   else
     fine_apply<1, double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
   timed_end(c, kind, s);
@@ -411,6 +450,8 @@ void apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const T* u, const
       fine_apply_node<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
     else if (P.nd == 3)
       fine_apply<3, T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+    else if (L.var == 2)
+      thermal_apply64<T>(L, mode, P.nc, P.lap, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
     else
       fine_apply<1, T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
     if (kind >= 0) timed_end(c, kind, s);
@@ -445,15 +486,18 @@ void setup_fp64(maple_ctx* c, Phys& P, const double* rho, hipStream_t s, bool fo
   P.v64.Ke = P.Ke;
   P.v64.rho = const_cast<double*>(rho);
   setup_t<double>(c, P, P.v64, rho, s, force_dense);
+  P.built[0] = true;
+}
+
+void setup_fp32(maple_ctx* c, Phys& P, const double* rho, hipStream_t s, bool force_dense) {
+  vec_copy<float, double>(c->lv[0].Ntot, rho, P.v32.rho, s);
+  setup_t<float>(c, P, P.v32, rho, s, force_dense);
+  P.built[1] = true;
 }
 
 void setup_hierarchy(maple_ctx* c, Phys& P, const double* rho, hipStream_t s) {
-  if (c->prec == MAPLE_PREC_FP64) {
-    setup_fp64(c, P, rho, s, false);
-  } else {
-    vec_copy<float, double>(c->lv[0].Ntot, rho, P.v32.rho, s);
-    setup_t<float>(c, P, P.v32, rho, s);
-  }
+  if (c->prec == MAPLE_PREC_FP64) setup_fp64(c, P, rho, s, false);
+  else setup_fp32(c, P, rho, s, false);
 }
 
 // z = M b at level l (symmetric V-cycle, zero start); returns the buffer holding the result.
@@ -528,6 +572,28 @@ void dot_rz(Phys& P, const double* r, const ZVec& z, double* partial, hipStream_
   else vec_dot<double, double>(P.ln[0], P.nd, P.nc, r, z.d, partial, s);
 }
 
+// (r, z) of every live case into SC_RZ_NEW and the per-component means of z into zmean
+// (the next direction is projected off the rigid translations)
+void dot_rz_mean(maple_ctx* c, Phys& P, const ZVec& z, hipStream_t s) {
+  double* zpart = P.partial + (size_t)c->lv[0].node.nblocks * 2 * MAPLE_MAX_CASES;
+  if (z.f) pcg_dot_rz<float>(P.ln[0], P.nd, P.nc, P.r, z.f, P.partial, zpart, s);
+  else pcg_dot_rz<double>(P.ln[0], P.nd, P.nc, P.r, z.d, P.partial, zpart, s);
+  reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RZ_NEW, s);
+  pcg_reduce_zmean(P.ln[0], P.nd, P.nc, zpart, P.sc, s);
+}
+
+// p = (z - zmean) + beta p
+void update_p(maple_ctx* c, Phys& P, const ZVec& z, hipStream_t s) {
+  (void)c;
+  if (z.f && pcg_update_p32(P.ln[0], P.nd, P.nc, P.sc, z.f, P.p, s) == 0) return;
+  if (z.f) {   // shapes without a fused kernel: widen first
+    vec_copy_cases<double, float>(P.ln[0], P.nd, P.nc, z.f, P.z64, s);
+    pcg_update_p(P.ln[0], P.nd, P.nc, P.sc, P.z64, P.p, s);
+  } else {
+    pcg_update_p(P.ln[0], P.nd, P.nc, P.sc, z.d, P.p, s);
+  }
+}
+
 // Launch tables over every cell (start of a solve).
 void full_launches(maple_ctx* c, Phys& P) {
   for (size_t l = 0; l < P.ln.size(); ++l) {
@@ -593,14 +659,14 @@ int poll_done(maple_ctx* c, Phys& P, int slot, bool* all) {
   HIPCHK(hipEventSynchronize(P.poll[slot]));
   const int n = c->ncells * P.nc;
   bool a = true;
-  for (int i = 0; i < n; ++i) a = a && P.done_h[slot * n + i];
+  for (int i = 0; i < n; ++i) a = a && P.done_h[slot * (n + 1) + i];
   *all = a;
   return 0;
 }
 
 int enqueue_poll(maple_ctx* c, Phys& P, int slot, hipStream_t s) {
   const int n = c->ncells * P.nc;
-  HIPCHK(hipMemcpyAsync(P.done_h + slot * n, P.done_d, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(P.done_h + slot * (n + 1), P.done_d, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, s));
   HIPCHK(hipEventRecord(P.poll[slot], s));
   return 0;
 }
@@ -609,10 +675,11 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   Level& L0 = c->lv[0];
   const bool flex = c->flexible < 0 ? c->prec != MAPLE_PREC_FP64 : c->flexible != 0;
   bool use64 = c->prec == MAPLE_PREC_FP64;
-  const long long n0 = (long long)P.nc * P.nd * L0.Ntot;
-  // second partial-sum area of the solve (the mixed update's fused flexible dot)
+  bool want64 = false;   // a CG restart was seen: finish on the FP64 V-cycle
+  // second partial-sum area of the solve (the mixed update's fused flexible dot); the
+  // third (dot_rz_mean's sums of z) follows it
   double* partial2 = P.partial + (size_t)c->lv[0].node.nblocks * MAPLE_MAX_CASES;
-  vec_zero<int>((long long)c->ncells * P.nc, P.done_d, s);   // no case is converged before the start
+  vec_zero<int>((long long)c->ncells * P.nc + 1, P.done_d, s);   // no case is converged before the start
   full_launches(c, P);
   fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, P.partial, s);
@@ -635,18 +702,16 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, P.partial, s);
   reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RR, s);
   ZVec z = precondition_cg(c, P, P.r, use64, false, s);
-  dot_rz(P, P.r, z, P.partial, s);
-  reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RZ_NEW, s);
-  pcg_init_scalars(c->ncells, P.nc, rtol, P.atol2, P.sc, P.done_d, s);
-  if (z.f) vec_copy<double, float>(n0, z.f, P.p, s);
-  else vec_copy<double, double>(n0, z.d, P.p, s);
+  dot_rz_mean(c, P, z, s);
+  pcg_init_scalars(c->ncells, P.nc, rtol, P.atol2, P.sc, P.done_d, s);   // beta = 0: p = z - zmean
+  update_p(c, P, z, s);
   TRY(enqueue_poll(c, P, 0, s));
   bool all = false;
   TRY(poll_done(c, P, 0, &all));
   for (int it = 0; it < maxit && !all; ++it) {
     apply_fine64(c, P, 1, rho, P.p, nullptr, P.q, P.partial, s);
     reduce_partials(op_launch(c, P, false), P.nc, P.partial, P.sc, SC_PQ, s);
-    pcg_alpha(c->ncells, P.nc, P.sc, s);
+    pcg_alpha(c->ncells, P.nc, P.sc, P.done_d, P.done_d + c->ncells * P.nc, s);
     // mixed form: the update also writes the FP32 preconditioner's input and, from
     // the previous FP32 z, the flexible dot (r_new, z_old)
     bool r32_ready = false, rzo_done = false;
@@ -657,7 +722,7 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
     } else {
       pcg_update_ur(P.ln[0], P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, P.partial, s);
     }
-    reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RR, s);
+    reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RR_LIVE, s);
     if (rzo_done) reduce_partials(P.ln[0], P.nc, partial2, P.sc, SC_RZO, s);
     pcg_check(c->ncells, P.nc, P.sc, P.done_d, s);
     const int slot = (it + 1) & 1;
@@ -670,32 +735,37 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
       // An FP32 V-cycle cannot resolve the soft modes of solid islands held by 1e-9
       // voids: a mixed solve still running after `fallback` iterations finishes on
       // the FP64 hierarchy of the same density (flexible CG absorbs the switch).
-      if (!use64 && c->fallback > 0 && it + 1 == c->fallback) {
+      // A restart (p^T K p <= 0 on a live case) under the FP32 preconditioner is the same
+      // failure seen earlier: switch at once.
+      if (!use64 && c->fallback > 0 && (it + 1 == c->fallback || want64)) {
         setup_fp64(c, P, rho, s, true);
         use64 = true;
       }
       z = precondition_cg(c, P, P.r, use64, r32_ready && !use64, s);
-      dot_rz(P, P.r, z, P.partial, s);
-      reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RZ_NEW, s);
+      dot_rz_mean(c, P, z, s);
       pcg_beta(c->ncells, P.nc, flex, P.sc, s);
-      if (!(z.f && pcg_update_p32(P.ln[0], P.nd, P.nc, P.sc, z.f, P.p, s) == 0)) {
-        if (z.f) {   // shapes without a fused kernel: widen first
-          vec_copy_cases<double, float>(P.ln[0], P.nd, P.nc, z.f, P.z64, s);
-          pcg_update_p(P.ln[0], P.nd, P.nc, P.sc, P.z64, P.p, s);
-        } else {
-          pcg_update_p(P.ln[0], P.nd, P.nc, P.sc, z.d, P.p, s);
-        }
-      }
+      update_p(c, P, z, s);
     }
     // the device is already working on the next iteration while we look at this one
     TRY(poll_done(c, P, slot, &all));
-    if (!all) TRY(compact_launches(c, P, P.done_h + slot * c->ncells * P.nc, s));
+    want64 = want64 || P.done_h[slot * (c->ncells * P.nc + 1) + c->ncells * P.nc] != 0;
+    if (!all) TRY(compact_launches(c, P, P.done_h + slot * (c->ncells * P.nc + 1), s));
   }
   P.have_fields = true;
   HIPCHK(hipMemcpyAsync(P.hsc.data(), P.sc, sizeof(CaseScalars) * P.hsc.size(), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  for (const CaseScalars& cs : P.hsc)
-    if (!(cs.rr == cs.rr) || !std::isfinite(cs.ff)) return fail(MAPLE_ENONFINITE, "non-finite residual in the cell solve");
+  for (size_t i = 0; i < P.hsc.size(); ++i) {
+    const CaseScalars& cs = P.hsc[i];
+    if (!(cs.rr == cs.rr) || !std::isfinite(cs.ff))
+      return fail(MAPLE_ENONFINITE, "non-finite residual in the cell solve (cell " + std::to_string(i / P.nc) +
+                                        ", load case " + std::to_string(i % P.nc) + ", after " +
+                                        std::to_string(cs.iters) + " iterations)");
+  }
+  for (size_t i = 0; i < P.hsc.size(); ++i)
+    if (P.hsc[i].brk)
+      return fail(MAPLE_EBREAKDOWN, "CG breakdown (p^T K p <= 0) in cell " + std::to_string(i / P.nc) + ", load case " +
+                                        std::to_string(i % P.nc) + " after " + std::to_string(P.hsc[i].iters) +
+                                        " iterations; its fields are not converged");
   return all ? 0 : MAPLE_ENOCONV;
 }
 
@@ -704,7 +774,7 @@ int bpcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipSt
   Level& L0 = c->lv[0];
   const Launch& LN = P.ln[0];
   bool use64 = c->prec == MAPLE_PREC_FP64;
-  vec_zero<int>((long long)c->ncells * P.nc, P.done_d, s);
+  vec_zero<int>((long long)c->ncells * P.nc + 1, P.done_d, s);
   full_launches(c, P);
   fine_rhs(L0.node, P.nd, P.nc, P.Fe, rho, P.f, s);
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, P.partial, s);
@@ -752,7 +822,7 @@ int bpcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipSt
       bcg_new_direction(LN, P.nd, P.nc, false, P.bs, z, P.p, P.partial, s);
     }
     TRY(poll_done(c, P, slot, &all));
-    if (!all) TRY(compact_launches(c, P, P.done_h + slot * c->ncells * P.nc, s));
+    if (!all) TRY(compact_launches(c, P, P.done_h + slot * (c->ncells * P.nc + 1), s));
   }
   P.have_fields = true;
   HIPCHK(hipMemcpyAsync(P.hsc.data(), P.sc, sizeof(CaseScalars) * P.hsc.size(), hipMemcpyDeviceToHost, s));
@@ -791,7 +861,10 @@ int solve_physics(maple_ctx* c, int p, const double* rho, double rtol, int maxit
   int status = solve_cases(c, P, rho, rtol, maxit, s);
   if (status == MAPLE_ENOCONV) {
     g_err = std::string(p == 0 ? "elastic" : "thermal") + " CG reached maxit before rtol";
+  } else if (status == MAPLE_EBREAKDOWN) {
+    g_err = std::string(p == 0 ? "elastic: " : "thermal: ") + g_err;
   } else if (status) {
+    if (status == MAPLE_ENONFINITE) g_err = std::string(p == 0 ? "elastic: " : "thermal: ") + g_err;
     return status;
   }
   element_energy(c->lv[0].node, P.nd, P.nc, P.Ke, P.X0, rho, P.u, P.Se, P.partial, s);
@@ -819,6 +892,9 @@ int find_islands(maple_ctx* c, const double* rho, hipStream_t s) {
   HIPCHK(hipGetLastError());
   return 0;
 }
+
+// Outcomes of a solve whose results are still written: maxit reached, CG breakdown.
+bool soft_status(int rc) { return rc == MAPLE_ENOCONV || rc == MAPLE_EBREAKDOWN; }
 
 int check_ctx(const maple_ctx* c) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
@@ -1019,6 +1095,11 @@ int maple_set_variant(maple_ctx* c, int kernel, int variant) {
     for (Phys& P : c->ph) P.lf.var = variant;
     return 0;
   }
+  if (kernel == MAPLE_KERNEL_THERMAL_OPERATOR) {
+    if (variant < 0 || variant > 2) return fail(MAPLE_EINVAL, "unknown variant");
+    c->thermal_var = variant;
+    return 0;
+  }
   if (kernel == MAPLE_KERNEL_COARSE_OPERATOR) {
     if (variant < 0 || variant > 2) return fail(MAPLE_EINVAL, "unknown variant");
     for (int l = 1; l < c->nlevels; ++l) {
@@ -1121,12 +1202,20 @@ int maple_homogenize(maple_ctx* c, const double* rho, int physics, double rtol, 
   const bool both = (physics & MAPLE_ELASTIC) && (physics & MAPLE_THERMAL);
   if (!both || !c->concurrent) {
     int status = 0;
+    std::string msg;
     for (int p = 0; p < 2; ++p) {
       if (!(physics & (p == 0 ? MAPLE_ELASTIC : MAPLE_THERMAL))) continue;
       const int rc = solve_physics(c, p, rho, rtol, maxit, p == 0 ? CH : kappa, iters_host, s);
-      if (rc == MAPLE_ENOCONV) status = rc;
-      else if (rc) return rc;
+      if (soft_status(rc)) {
+        if (status != MAPLE_EBREAKDOWN) {
+          status = rc;
+          msg = g_err;
+        }
+      } else if (rc) {
+        return rc;
+      }
     }
+    if (status) g_err = msg;
     return status;
   }
   // The two physics share nothing but rho: the thermal solve runs on a second
@@ -1150,12 +1239,11 @@ int maple_homogenize(maple_ctx* c, const double* rho, int physics, double rtol, 
   th.join();
   HIPCHK(hipEventRecord(c->join, c->side));
   HIPCHK(hipStreamWaitEvent(s, c->join, 0));
-  if (rc_el && rc_el != MAPLE_ENOCONV) return fail(rc_el, err_el);
-  if (rc_th && rc_th != MAPLE_ENOCONV) return fail(rc_th, err_th);
-  if (rc_el || rc_th) {
-    g_err = rc_el ? err_el : err_th;
-    return MAPLE_ENOCONV;
-  }
+  if (rc_el && !soft_status(rc_el)) return fail(rc_el, err_el);
+  if (rc_th && !soft_status(rc_th)) return fail(rc_th, err_th);
+  // results are written for both soft outcomes; a breakdown outranks maxit
+  if (rc_el == MAPLE_EBREAKDOWN || (rc_el && rc_th != MAPLE_EBREAKDOWN)) return fail(rc_el, err_el);
+  if (rc_th) return fail(rc_th, err_th);
   return 0;
 }
 
@@ -1337,6 +1425,161 @@ int maple_timing_read(maple_ctx* c, int kind, long long* count, double* total_ms
   HIPCHK(hipMemset(c->work_d + kind, 0, sizeof(w)));
   if (work) *work = (long long)w;
   return 0;
+}
+
+}  // extern "C"
+
+// ---- multigrid internals ----------------------------------------------------
+
+namespace {
+
+int mg_args(maple_ctx* c, int physics, int prec, Phys** P) {
+  TRY(check_ctx(c));
+  if (physics != MAPLE_ELASTIC && physics != MAPLE_THERMAL) return fail(MAPLE_EINVAL, "physics must be one of ELASTIC, THERMAL");
+  if (prec != MAPLE_PREC_FP64 && prec != MAPLE_PREC_MIXED) return fail(MAPLE_EINVAL, "unknown precision");
+  if (prec == MAPLE_PREC_MIXED && c->prec != MAPLE_PREC_MIXED)
+    return fail(MAPLE_EINVAL, "the FP32 hierarchy exists in MAPLE_PREC_MIXED contexts only");
+  *P = &c->ph[physics == MAPLE_ELASTIC ? 0 : 1];
+  return 0;
+}
+
+int mg_built(const Phys& P, int prec) {
+  if (!P.built[prec == MAPLE_PREC_FP64 ? 0 : 1]) return fail(MAPLE_EINVAL, "no hierarchy of this precision yet (maple_mg_setup)");
+  return 0;
+}
+
+// Launch tables over every cell and case (the last solve's convergence flags cleared).
+void mg_full(maple_ctx* c, Phys& P, hipStream_t s) {
+  vec_zero<int>((long long)c->ncells * P.nc, P.done_d, s);
+  full_launches(c, P);
+}
+
+template <typename T>
+int mg_apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const double* u, const double* f, double* out,
+               hipStream_t s) {
+  const long long n = (long long)P.nc * P.nd * c->lv[l].Ntot;
+  if constexpr (std::is_same<T, double>::value) {
+    apply_t<double>(c, P, V, l, mode, u, f, out, s);
+  } else {
+    // FP32 staging in the (unused between solves) FP64 CG vectors, which hold twice as many floats
+    float* u32 = (float*)P.p;
+    float* f32 = (float*)P.q;
+    float* o32 = (float*)P.r;
+    vec_copy<float, double>(n, u, u32, s);
+    if (f) vec_copy<float, double>(n, f, f32, s);
+    apply_t<float>(c, P, V, l, mode, u32, f ? f32 : nullptr, o32, s);
+    vec_copy<double, float>(n, o32, out, s);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int maple_mg_setup(maple_ctx* c, int physics, const double* rho, int prec, void* stream) {
+  Phys* P;
+  TRY(mg_args(c, physics, prec, &P));
+  if (!rho) return fail(MAPLE_EINVAL, "rho is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == MAPLE_PREC_FP64) setup_fp64(c, *P, rho, s, true);
+  else setup_fp32(c, *P, rho, s, true);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int maple_mg_stencil(maple_ctx* c, int physics, int prec, int level, double* S, void* stream) {
+  Phys* P;
+  TRY(mg_args(c, physics, prec, &P));
+  TRY(mg_built(*P, prec));
+  if (level < 1 || level >= c->nlevels) return fail(MAPLE_EINVAL, "stencils exist on levels 1 .. nlevels-1");
+  if (!S) return fail(MAPLE_EINVAL, "null output");
+  const long long n = 27LL * P->nd * P->nd * c->lv[level].Ntot;
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == MAPLE_PREC_FP64) vec_copy<double, double>(n, P->v64.S[level], S, s);
+  else vec_copy<double, float>(n, P->v32.S[level], S, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int maple_mg_dinv(maple_ctx* c, int physics, int prec, int level, double* dinv, void* stream) {
+  Phys* P;
+  TRY(mg_args(c, physics, prec, &P));
+  TRY(mg_built(*P, prec));
+  if (level < 0 || level >= c->nlevels - 1) return fail(MAPLE_EINVAL, "smoother diagonals exist on levels 0 .. nlevels-2");
+  if (!dinv) return fail(MAPLE_EINVAL, "null output");
+  const long long n = (long long)P->nd * c->lv[level].Ntot;
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == MAPLE_PREC_FP64) vec_copy<double, double>(n, P->v64.dinv[level], dinv, s);
+  else vec_copy<double, float>(n, P->v32.dinv[level], dinv, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int maple_mg_apply(maple_ctx* c, int physics, int prec, int level, int mode, const double* u, const double* f,
+                   double* out, void* stream) {
+  Phys* P;
+  TRY(mg_args(c, physics, prec, &P));
+  TRY(mg_built(*P, prec));
+  if (level < 0 || level >= c->nlevels) return fail(MAPLE_EINVAL, "level out of range");
+  if (mode != 0 && mode != 2 && mode != 3) return fail(MAPLE_EINVAL, "mode must be 0 (A u), 2 (f - A u) or 3 (Jacobi sweep)");
+  if (mode == 3 && level == c->nlevels - 1) return fail(MAPLE_EINVAL, "the coarsest level has no smoother");
+  if (!u || !out || (mode != 0 && !f)) return fail(MAPLE_EINVAL, "null argument");
+  if (u == out || (f && f == out)) return fail(MAPLE_EINVAL, "out may not alias u or f");
+  hipStream_t s = (hipStream_t)stream;
+  mg_full(c, *P, s);
+  if (prec == MAPLE_PREC_FP64) return mg_apply_t<double>(c, *P, P->v64, level, mode, u, mode ? f : nullptr, out, s);
+  return mg_apply_t<float>(c, *P, P->v32, level, mode, u, mode ? f : nullptr, out, s);
+}
+
+int maple_mg_restrict(maple_ctx* c, int physics, int level, const double* fine, double* coarse, void* stream) {
+  Phys* P;
+  TRY(mg_args(c, physics, MAPLE_PREC_FP64, &P));
+  if (level < 1 || level >= c->nlevels) return fail(MAPLE_EINVAL, "level must be in [1, nlevels)");
+  if (!fine || !coarse) return fail(MAPLE_EINVAL, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  mg_full(c, *P, s);
+  restrict_pt<double>(P->ln[level], P->ln[level - 1], P->nd, P->nc, fine, coarse, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int maple_mg_vcycle(maple_ctx* c, int physics, int prec, const double* r, double* z, void* stream) {
+  Phys* P;
+  TRY(mg_args(c, physics, prec, &P));
+  TRY(mg_built(*P, prec));
+  if (!r || !z || r == z) return fail(MAPLE_EINVAL, "need distinct r and z");
+  hipStream_t s = (hipStream_t)stream;
+  mg_full(c, *P, s);
+  const long long n = (long long)P->nc * P->nd * c->lv[0].Ntot;
+  if (prec == MAPLE_PREC_FP64) {
+    double* zz = vcycle_t<double>(c, *P, P->v64, 0, r, s);
+    vec_copy<double, double>(n, zz, z, s);
+  } else {
+    vec_copy<float, double>(n, r, P->v32.B[0], s);
+    float* zz = vcycle_t<float>(c, *P, P->v32, 0, P->v32.B[0], s);
+    vec_copy<double, float>(n, zz, z, s);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int maple_mg_coarsest_inverse(maple_ctx* c, int physics, int cell, double* Ainv, void* stream) {
+  Phys* P;
+  TRY(mg_args(c, physics, MAPLE_PREC_FP64, &P));
+  if (!P->built[0] && !P->built[1]) return fail(MAPLE_EINVAL, "no hierarchy yet (maple_mg_setup)");
+  if (cell < 0 || cell >= c->ncells) return fail(MAPLE_EINVAL, "cell index out of range");
+  if (!Ainv) return fail(MAPLE_EINVAL, "null output");
+  const DenseCell& dc = P->hdc[cell];
+  HIPCHK(hipMemcpyAsync(Ainv, P->A + dc.aoff, sizeof(double) * dc.n * dc.n, hipMemcpyDeviceToDevice,
+                        (hipStream_t)stream));
+  return 0;
+}
+
+int maple_mg_coarsest_order(const maple_ctx* c, int physics, int cell) {
+  if (!c || cell < 0 || cell >= c->ncells || (physics != MAPLE_ELASTIC && physics != MAPLE_THERMAL)) return 0;
+  return c->ph[physics == MAPLE_ELASTIC ? 0 : 1].hdc[cell].n;
 }
 
 }  // extern "C"

@@ -65,6 +65,12 @@ void fine_apply_nodeb(const Launch& L, int mode, int ncase, const T* Ke, const T
 template <typename T>
 void fine_apply_nodeb64(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u,
                         const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s);
+// Thermal operator in node form for nz = 64 cells whose element matrix is the Q1
+// Laplacian (entry k[d] for corners at Hamming distance d; coef: 4 per cell), same
+// modes, all (<= 3) load cases per thread; L: the pair launch.
+template <typename T>
+void thermal_apply64(const Launch& L, int mode, int ncase, const double* coef, const T* rho, const T* u, const T* f,
+                     const T* dinv, T omega, T* out, double* partial, hipStream_t s);
 void fine_rhs(const Launch& L, int nd, int ncase, const double* Fe, const double* rho, double* F, hipStream_t s);
 template <typename T>
 void fine_dinv(const Launch& L, int nd, bool l1, const double* Ke, const double* rho, T* dinv, hipStream_t s);
@@ -117,8 +123,12 @@ struct CaseScalars {   // one per (cell, case)
   double alpha, beta;
   int done, iters;
   int src;             // start of the last solve: 0 zero, 1 previous fields, 2 supplied start
+  int brk;             // CG breakdown: p^T K p <= 0 (or NaN) on MAPLE_CG_RESTARTS directions in a row
+  int nrst;            // consecutive restarts (p^T K p <= 0): the next direction is p = z
   int pad_;
+  double zmean[3];     // per component: mean of the preconditioned residual over the cell's nodes
 };
+#define MAPLE_CG_RESTARTS 3
 // Block CG state of one cell (bcg.hip): Gram matrices of the current step (two
 // NC x NC, row-major, at 0 and 36), the step matrices, the Cholesky factor of
 // P^T K P, the orthonormalising transform and the active directions.
@@ -150,7 +160,9 @@ void bcg_init(int ncells, int nc, BlockScalars* bs, CaseScalars* sc, int* done_f
 // sum partial[blk*6 + j] over each cell's blocks into dst (a member of CaseScalars)
 // (over the cells the launch lists, with the launch that wrote the partials)
 void reduce_partials(const Launch& L, int ncase, const double* partial, CaseScalars* sc, int which, hipStream_t s);
-enum { SC_FF = 0, SC_RZ_NEW, SC_PQ, SC_RR, SC_RZO, SC_RRS };
+// SC_RR_LIVE: as SC_RR, but only for (cell, case) pairs not yet stopped (a stopped
+// pair's last residual stays; the update passes skip it and sum zero for it)
+enum { SC_FF = 0, SC_RZ_NEW, SC_PQ, SC_RR, SC_RZO, SC_RRS, SC_RR_LIVE };
 // elementwise (cell-aligned 256-node blocks):
 template <typename T>
 void vec_scale_dinv(const Launch& L, int nd, int ncase, T omega, const T* dinv, const T* b, T* x, hipStream_t s);
@@ -165,7 +177,16 @@ void pcg_update_ur(const Launch& L, int nd, int ncase, const CaseScalars* sc, co
 // supplied start us (residual rs, if have_start); u and r receive the choice.
 void pcg_pick_start(const Launch& L, int ncells, int nd, int ncase, bool have_prev, bool have_start, CaseScalars* sc,
                     double* u, double* r, const double* us, const double* rs, const double* f, hipStream_t s);
-// p = z + beta p
+// (r_j, z_j) into partial[blk*6 + j], and the sums of z over the block's nodes per (case,
+// component) into zpart[blk*18 + j*nd + i]
+template <typename Tz>
+void pcg_dot_rz(const Launch& L, int nd, int ncase, const double* r, const Tz* z, double* partial, double* zpart,
+                hipStream_t s);
+// sc.zmean of every (cell, case) the launch lists: its zpart sums over the cell's blocks / nodes
+void pcg_reduce_zmean(const Launch& L, int nd, int ncase, const double* zpart, CaseScalars* sc, hipStream_t s);
+// p = (z - zmean) + beta p (p not read where beta = 0): the search directions stay free of
+// the rigid translations, the null space of the periodic operator, so p^T K p measures
+// the direction's energy instead of the rounding of a translation
 void pcg_update_p(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* z, double* p, hipStream_t s);
 // mixed form, (nd, ncase) = (3, 6) or (1, 3) only (else -1, nothing launched): the
 // update also writes r32 = (float) r and, if zold is not null, partial2 = (r, zold)
@@ -177,7 +198,11 @@ int pcg_update_p32(const Launch& L, int nd, int ncase, const CaseScalars* sc, co
 // per (cell, case) scalar steps
 void pcg_init_scalars(int ncells, int ncase, double rtol, const double* atol2, CaseScalars* sc, int* done_flags,
                       hipStream_t s);
-void pcg_alpha(int ncells, int ncase, CaseScalars* sc, hipStream_t s);
+// alpha = rz / pq.  A live case with pq <= 0 (or NaN) takes no step and restarts
+// from p = z (beta = 0) and raises *restart_flag; after MAPLE_CG_RESTARTS such
+// directions in a row it is a breakdown: stopped (done, brk) with its fields and
+// residual as they are
+void pcg_alpha(int ncells, int ncase, CaseScalars* sc, int* done_flags, int* restart_flag, hipStream_t s);
 void pcg_check(int ncells, int ncase, CaseScalars* sc, int* done_flags, hipStream_t s);
 // flexible: beta = (rz_new - rzo) / rz (valid for a variable or slightly non-symmetric preconditioner)
 void pcg_beta(int ncells, int ncase, bool flexible, CaseScalars* sc, hipStream_t s);

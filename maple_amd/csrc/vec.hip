@@ -23,6 +23,7 @@ __global__ __launch_bounds__(64) void k_reduce_partials(const LevelCell* __restr
     else if (which == SC_PQ) s.pq = v;
     else if (which == SC_RZO) s.rzo = v;
     else if (which == SC_RRS) s.rrs = v;
+    else if (which == SC_RR_LIVE) { if (!s.done) s.rr = v; }
     else s.rr = v;
   }
 }
@@ -136,6 +137,85 @@ template void vec_dot<double, double>(const Launch&, int, int, const double*, co
 template void vec_dot<double, float>(const Launch&, int, int, const double*, const float*, double*, hipStream_t);
 template void vec_dot<float, float>(const Launch&, int, int, const float*, const float*, double*, hipStream_t);
 
+// (r, z) per case and the sums of z per (case, component) in one pass
+template <int ND, int NC, typename Tz>
+__global__ __launch_bounds__(256) void k_dot_rz_t(Launch L, const double* __restrict__ r, const Tz* __restrict__ z,
+                                                  double* __restrict__ partial, double* __restrict__ zpart) {
+  int ci; long long gid = 0;
+  const bool ok = vnode(L, ci, gid);
+  double v[NC], w[NC * ND];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    v[j] = 0.0;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) w[j * ND + i] = 0.0;
+    if (ok && !case_done(L.done, L.ndc, ci, j))
+#pragma unroll
+      for (int i = 0; i < ND; ++i) {
+        const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
+        const double zv = (double)z[idx];
+        v[j] += r[idx] * zv;
+        w[j * ND + i] = zv;
+      }
+  }
+  block_sum<NC>(v, partial + (long long)lbid() * MAPLE_MAX_CASES);
+  block_sum<NC * ND>(w, zpart + (long long)lbid() * 18);
+}
+
+template <typename Tz>
+__global__ __launch_bounds__(256) void k_dot_rz(Launch L, int nd, int ncase, const double* __restrict__ r,
+                                                const Tz* __restrict__ z, double* __restrict__ partial,
+                                                double* __restrict__ zpart) {
+  int ci; long long gid = 0;
+  const bool ok = vnode(L, ci, gid);
+  for (int j = 0; j < ncase; ++j) {
+    double v[1] = {0.0};
+    double w[3] = {0.0, 0.0, 0.0};
+    if (ok && !case_done(L.done, L.ndc, ci, j))
+      for (int i = 0; i < nd; ++i) {
+        const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
+        const double zv = (double)z[idx];
+        v[0] += r[idx] * zv;
+        w[i] = zv;
+      }
+    block_sum<1>(v, partial + (long long)lbid() * MAPLE_MAX_CASES + j);
+    block_sum<3>(w, zpart + (long long)lbid() * 18 + j * 3);
+  }
+}
+
+template <typename Tz>
+void pcg_dot_rz(const Launch& L, int nd, int ncase, const double* r, const Tz* z, double* partial, double* zpart,
+                hipStream_t s) {
+  if (nd == 3 && ncase == 6) k_dot_rz_t<3, 6, Tz><<<L.nblocks, 256, 0, s>>>(L, r, z, partial, zpart);
+  else if (nd == 1 && ncase == 3) k_dot_rz_t<1, 3, Tz><<<L.nblocks, 256, 0, s>>>(L, r, z, partial, zpart);
+  else k_dot_rz<Tz><<<L.nblocks, 256, 0, s>>>(L, nd, ncase, r, z, partial, zpart);
+}
+template void pcg_dot_rz<double>(const Launch&, int, int, const double*, const double*, double*, double*, hipStream_t);
+template void pcg_dot_rz<float>(const Launch&, int, int, const double*, const float*, double*, double*, hipStream_t);
+
+// one wavefront per listed (cell, case); the generic dot kernel stores its sums with a
+// stride of 3 per case, the unrolled one with nd
+__global__ __launch_bounds__(64) void k_reduce_zmean(const LevelCell* __restrict__ cells,
+                                                     const int* __restrict__ blk_start, int nd, int ncase,
+                                                     int stride, const double* __restrict__ zpart,
+                                                     CaseScalars* __restrict__ sc) {
+  const int k = blockIdx.x, j = blockIdx.y;
+  const LevelCell c = cells[k];
+  const int b0 = blk_start[k], b1 = blk_start[k + 1];
+  const double inv = 1.0 / ((double)c.nx * c.ny * c.nz);
+  for (int i = 0; i < nd; ++i) {
+    double v = 0.0;
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += 64) v += zpart[(long long)b * 18 + j * stride + i];
+    v = wave_sum(v);
+    if (threadIdx.x == 0) sc[c.id * ncase + j].zmean[i] = v * inv;
+  }
+}
+
+void pcg_reduce_zmean(const Launch& L, int nd, int ncase, const double* zpart, CaseScalars* sc, hipStream_t s) {
+  const bool unrolled = (nd == 3 && ncase == 6) || (nd == 1 && ncase == 3);
+  k_reduce_zmean<<<dim3(L.ncells, ncase), 64, 0, s>>>(L.cells, L.blk_start, nd, ncase, unrolled ? nd : 3, zpart, sc);
+}
+
 template <int ND, int NC>
 __global__ __launch_bounds__(256) void k_update_ur_t(Launch L, const CaseScalars* __restrict__ sc,
                                                      const double* __restrict__ p, const double* __restrict__ q,
@@ -148,7 +228,9 @@ __global__ __launch_bounds__(256) void k_update_ur_t(Launch L, const CaseScalars
   for (int j = 0; j < NC; ++j) {
     v[j] = 0.0;
     const double al = sc[ci * NC + j].alpha;
-    if (ok && al != 0.0) {   // a converged case keeps u and r untouched (its p may be stale)
+    // a stopped case keeps u and r untouched (its p may be stale); a live one with
+    // alpha = 0 (a restart) keeps them too but still sums its residual norm
+    if (ok && !case_done(L.done, L.ndc, ci, j)) {
 #pragma unroll
       for (int i = 0; i < ND; ++i) {
         const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
@@ -171,7 +253,7 @@ __global__ __launch_bounds__(256) void k_update_ur(Launch L, int nd, int ncase, 
   for (int j = 0; j < ncase; ++j) {
     double v[1] = {0.0};
     const double al = ok ? sc[ci * ncase + j].alpha : 0.0;
-    if (ok && al != 0.0) {
+    if (ok && !case_done(L.done, L.ndc, ci, j)) {
       for (int i = 0; i < nd; ++i) {
         const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
         u[idx] = fma(al, p[idx], u[idx]);
@@ -201,7 +283,7 @@ __global__ __launch_bounds__(256) void k_update_ur_mixed(Launch L, const CaseSca
   for (int j = 0; j < NC; ++j) {
     v[j] = w[j] = 0.0;
     const double al = sc[ci * NC + j].alpha;
-    if (ok && al != 0.0) {   // a converged case keeps u, r and r32 untouched
+    if (ok && !case_done(L.done, L.ndc, ci, j)) {   // a stopped case keeps u, r and r32 untouched
 #pragma unroll
       for (int i = 0; i < ND; ++i) {
         const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
@@ -244,12 +326,14 @@ __global__ __launch_bounds__(256) void k_update_p_t(Launch L, const CaseScalars*
   if (!vnode(L, ci, gid)) return;
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
-    if (sc[ci * NC + j].done) continue;   // a converged case's p is never used again
-    const double be = sc[ci * NC + j].beta;
+    const CaseScalars& cs = sc[ci * NC + j];
+    if (cs.done) continue;   // a converged case's p is never used again
+    const double be = cs.beta;
 #pragma unroll
     for (int i = 0; i < ND; ++i) {
       const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
-      p[idx] = fma(be, p[idx], (double)z[idx]);
+      const double zc = (double)z[idx] - cs.zmean[i];
+      p[idx] = be == 0.0 ? zc : fma(be, p[idx], zc);
     }
   }
 }
@@ -259,11 +343,13 @@ __global__ __launch_bounds__(256) void k_update_p(Launch L, int nd, int ncase, c
   int ci; long long gid;
   if (!vnode(L, ci, gid)) return;
   for (int j = 0; j < ncase; ++j) {
-    if (sc[ci * ncase + j].done) continue;
-    const double be = sc[ci * ncase + j].beta;
+    const CaseScalars& cs = sc[ci * ncase + j];
+    if (cs.done) continue;
+    const double be = cs.beta;
     for (int i = 0; i < nd; ++i) {
       const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
-      p[idx] = fma(be, p[idx], z[idx]);
+      const double zc = z[idx] - cs.zmean[i];
+      p[idx] = be == 0.0 ? zc : fma(be, p[idx], zc);
     }
   }
 }
@@ -331,6 +417,8 @@ __global__ void k_pcg_init(int n, int ncase, double rtol, const double* __restri
   s.tgt2 = fmax(rtol * rtol * s.ff, atol2[t / ncase]);
   s.done = s.rr <= s.tgt2 ? 1 : 0;
   s.iters = 0;
+  s.brk = 0;
+  s.nrst = 0;
   s.alpha = s.beta = 0.0;
   s.rz = s.rz_new;
   done_flags[t] = s.done;
@@ -342,16 +430,37 @@ void pcg_init_scalars(int ncells, int ncase, double rtol, const double* atol2, C
   k_pcg_init<<<(n + 63) / 64, 64, 0, s>>>(n, ncase, rtol, atol2, sc, done_flags);
 }
 
-__global__ void k_pcg_alpha(int n, CaseScalars* __restrict__ sc) {
+__global__ void k_pcg_alpha(int n, CaseScalars* __restrict__ sc, int* __restrict__ done_flags,
+                            int* __restrict__ restart_flag) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   CaseScalars& s = sc[t];
-  s.alpha = (!s.done && s.pq > 0.0) ? s.rz / s.pq : 0.0;
+  if (s.done) {
+    s.alpha = 0.0;
+    return;
+  }
+  if (s.pq > 0.0) {
+    s.alpha = s.rz / s.pq;
+    s.nrst = 0;
+    return;
+  }
+  // p^T K p <= 0 (or NaN) on a live case: the preconditioned direction is not a descent
+  // direction of this operator (a preconditioner that is not SPD there, or a direction
+  // lost in the operator's null space).  No step; the next direction is p = z.
+  s.alpha = 0.0;
+  s.nrst += 1;
+  if (s.nrst >= MAPLE_CG_RESTARTS) {
+    s.brk = 1;
+    s.done = 1;
+    done_flags[t] = 1;
+  } else {
+    *restart_flag = 1;
+  }
 }
 
-void pcg_alpha(int ncells, int ncase, CaseScalars* sc, hipStream_t s) {
+void pcg_alpha(int ncells, int ncase, CaseScalars* sc, int* done_flags, int* restart_flag, hipStream_t s) {
   const int n = ncells * ncase;
-  k_pcg_alpha<<<(n + 63) / 64, 64, 0, s>>>(n, sc);
+  k_pcg_alpha<<<(n + 63) / 64, 64, 0, s>>>(n, sc, done_flags, restart_flag);
 }
 
 __global__ void k_pcg_check(int n, CaseScalars* __restrict__ sc, int* __restrict__ done_flags) {
@@ -375,7 +484,8 @@ __global__ void k_pcg_beta(int n, int flexible, CaseScalars* __restrict__ sc) {
   if (t >= n) return;
   CaseScalars& s = sc[t];
   const double num = flexible ? s.rz_new - s.rzo : s.rz_new;
-  s.beta = (!s.done && s.rz != 0.0) ? num / s.rz : 0.0;
+  // a restarted case (no step this iteration) starts over from p = z
+  s.beta = (!s.done && s.nrst == 0 && s.rz != 0.0) ? num / s.rz : 0.0;
   s.rz = s.rz_new;
 }
 

@@ -171,10 +171,14 @@ class OCDriver:
 
     def __init__(self, cells: CellBatch, objectives: Sequence, volfrac: Sequence[float], penal: float = 3.0,
                  rho_min: float = 1e-9, move: float = 0.2, xmin: float = 1e-3, filter_k: int = 3,
-                 rtol: float = 1e-6, maxit: int = 1000, physics: Optional[Sequence[str]] = None, start=None):
+                 rtol: float = 1e-6, maxit: int = 1000, physics: Optional[Sequence[str]] = None, start=None,
+                 strict: bool = True):
         """physics: solve these physics whatever the objectives need (None: what they need).
         start: callable(rho, physics) that supplies start candidates before each
-        homogenization (e.g. maple_amd.learned.LearnedStart)."""
+        homogenization (e.g. maple_amd.learned.LearnedStart).
+        strict: raise when a solve stops before rtol (maxit or CG breakdown); with
+        strict=False the step goes on with the fields reached and reports the (cell,
+        case) pairs that stopped early in info["unconverged"]."""
         self.cells = cells
         n = cells.ncells
         objs = list(objectives) if not isinstance(objectives, (str, dict)) else [objectives] * n
@@ -192,6 +196,7 @@ class OCDriver:
             if (self.need_el and "elastic" not in self.physics) or (self.need_th and "thermal" not in self.physics):
                 raise ValueError("the objectives need a physics that is not solved")
         self.start = start
+        self.strict = bool(strict)
 
     def design_to_rho(self, x: torch.Tensor) -> torch.Tensor:
         xu = x.clamp(self.xmin, 1.0)
@@ -203,7 +208,9 @@ class OCDriver:
         rho = self.rho_min + (1.0 - self.rho_min) * xu ** self.penal
         if self.start is not None:
             self.start(rho, self.physics)
-        CH, kap, iters = self.cells.homogenize(rho, self.physics, rtol=self.rtol, maxit=self.maxit)
+        CH, kap, iters = self.cells.homogenize(rho, self.physics, rtol=self.rtol, maxit=self.maxit,
+                                               strict=self.strict)
+        status = self.cells.last_status
         CHh = CH.cpu().numpy() if CH is not None else None
         kh = kap.cpu().numpy() if kap is not None else None
         n = self.cells.ncells
@@ -219,4 +226,7 @@ class OCDriver:
         if self.filter_k > 1:
             dJdx = self.cells.filter(dJdx, self.filter_k)
         xnew = self.cells.oc_update(x, dJdx, self.volfrac, self.move, self.xmin)
-        return xnew, {"J": J, "CH": CHh, "kappa": kh, "iters": iters.copy(), "rho": rho, "dJdx": dJdx}
+        cols = [k for p, ks in (("elastic", range(6)), ("thermal", range(6, 9))) if p in self.physics for k in ks]
+        unconv = int((iters[:, cols] >= self.maxit).sum()) if status else 0
+        return xnew, {"J": J, "CH": CHh, "kappa": kh, "iters": iters.copy(), "rho": rho, "dJdx": dJdx,
+                      "dJdrho": dJdrho, "G6": G6, "G3": G3, "status": int(status), "unconverged": unconv}

@@ -57,25 +57,28 @@ class CellSolver:
         self.maxit = maxit
         self.iters = {}
 
-    def solve(self, rho, physics: str, rtol=None):
+    def solve(self, rho, physics: str, rtol=None, x0=None):
+        """x0 (N, nd, ncase): CG start.  The result is whatever meets this solver's own
+        residual test ||f - K u|| <= rtol ||f|| on its own operator, from any start; a
+        start only changes how many iterations that takes."""
         rho = np.asarray(rho, dtype=np.float64).ravel()
         Ke, Fe = (self.Ke3, self.Fe3) if physics == "elastic" else (self.Ke1, self.Fe1)
         H = Hierarchy(rho, Ke, self.dims, self.nlevels, omega=self.omega, nsmooth=self.nsmooth)
         F = assemble_rhs(rho, Fe, self.dims)
         fscale = np.linalg.norm(Fe) * np.sqrt(rho.size) * np.max(np.abs(rho))
-        u, it = pcg(H, F, rtol=self.rtol if rtol is None else rtol, maxit=self.maxit,
+        u, it = pcg(H, F, rtol=self.rtol if rtol is None else rtol, maxit=self.maxit, x0=x0,
                     fscale=fscale)
         self.iters[physics] = it
         return u
 
-    def elastic(self, rho, rtol=None):
+    def elastic(self, rho, rtol=None, x0=None):
         """Returns (C_H, S_e, u)."""
-        u = self.solve(rho, "elastic", rtol)
+        u = self.solve(rho, "elastic", rtol, x0)
         S = element_energies(u, self.Ke3, self.X03, self.dims)
         return homogenize(rho, S), S, u
 
-    def thermal(self, rho, rtol=None):
+    def thermal(self, rho, rtol=None, x0=None):
         """Returns (kappa_H, S_e, u)."""
-        u = self.solve(rho, "thermal", rtol)
+        u = self.solve(rho, "thermal", rtol, x0)
         S = element_energies(u, self.Ke1, self.X01, self.dims)
         return homogenize(rho, S), S, u

@@ -378,3 +378,47 @@ def test_concurrent_physics_equal_sequential():
     da = a.sensitivity(torch.ones(2, 6, 6, dtype=torch.float64, device=DEV), torch.ones(2, 3, 3, dtype=torch.float64, device=DEV))
     db = b.sensitivity(torch.ones(2, 6, 6, dtype=torch.float64, device=DEV), torch.ones(2, 3, 3, dtype=torch.float64, device=DEV))
     assert torch.equal(da, db)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("r,ncase", [(64, 3), (64, 2), (32, 3)])
+def test_thermal_operator_variants(r, ncase, variant):
+    """The thermal level-0 operator: element rows (variants 0, 1) and the node form
+    (variant 2; nz = 64 cells with a Q1-Laplacian element matrix, element rows
+    otherwise) against the oracle's EBE product, also for fewer load cases than three."""
+    import maple_amd
+    print(f"[thermal variant] r={r} variant={variant} ncase={ncase}", flush=True)
+    cb = _cells([r], 4)
+    cb.set_variant(maple_amd._lib.MAPLE_KERNEL_THERMAL_OPERATOR, variant)
+    rho = _rho(r, 7 + variant)
+    u = np.random.default_rng(variant).standard_normal((ncase, 1, r ** 3))
+    v = cb.apply_operator("thermal", _t(rho), _t(u)).cpu().numpy()
+    Ke = ofe.hexahedron_scalar(r)[0]
+    for j in range(ncase):
+        ref = ogrid.ebe_matvec(u[j].T.copy(), rho, Ke, (r, r, r)).T
+        assert np.abs(v[j] - ref).max() <= 1e-13 * np.abs(ref).max()
+    cb.close()
+
+
+def test_thermal_operator_general_element_matrix():
+    """An element matrix without the Q1-Laplacian structure (one coupling of corners 0
+    and 1 shifted, still symmetric with zero row sums) falls back to the element-row
+    kernel and stays exact."""
+    print("[thermal general Ke]", flush=True)
+    r = 64
+    cb = _cells([r], 4)
+    Ke, Fe, X0 = ofe.hexahedron_scalar(r)
+    Ka = Ke.copy()
+    d = 0.01 * Ke[0, 0]
+    Ka[0, 1] += d
+    Ka[1, 0] += d
+    Ka[0, 0] -= d
+    Ka[1, 1] -= d
+    cb.set_element_matrices(0, Ke8=Ka, Fe8=Fe, X0_8=X0)
+    rho = _rho(r, 3)
+    u = np.random.default_rng(5).standard_normal((3, 1, r ** 3))
+    v = cb.apply_operator("thermal", _t(rho), _t(u)).cpu().numpy()
+    for j in range(3):
+        ref = ogrid.ebe_matvec(u[j].T.copy(), rho, Ka, (r, r, r)).T
+        assert np.abs(v[j] - ref).max() <= 1e-13 * np.abs(ref).max()
+    cb.close()
