@@ -211,6 +211,24 @@ int maple_symmetrize(maple_ctx* ctx, int kind, int axes, const double* in_dev, d
 int maple_oc_update(maple_ctx* ctx, const double* x_dev, const double* dJdx_dev, const double* volfrac_host,
                     const double* move_host, const double* xmin_host, int nbisect, double* xnew_dev, void* stream);
 
+/* The OC update of maple_oc_update without a context and without allocating:
+ * no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  `ncells` cells with node grids dims_host[3*i..3*i+2] on
+ * `device`, arrays as in maple_oc_update (x, dJdx, xnew: device, all cells
+ * concatenated; volfrac/move/xmin: per-cell host arrays).  The scratch (bisection
+ * state, partial sums, launch tables, about 4.3 MB) is a static device array that
+ * each device holds until the process ends; calls on one device share it, so calls
+ * that overlap on one device (two streams or threads) race.  At most
+ * MAPLE_NOALLOC_MAX_CELLS cells and MAPLE_NOALLOC_MAX_ELEMENTS elements in all
+ * (else MAPLE_EINVAL, nothing launched).  Same results as maple_oc_update, bit
+ * for bit. */
+#define MAPLE_NOALLOC_MAX_CELLS 64
+#define MAPLE_NOALLOC_MAX_ELEMENTS 8388608LL
+int maple_oc_update_noalloc(int device, int ncells, const int* dims_host, const double* x_dev, const double* dJdx_dev,
+                            const double* volfrac_host, const double* move_host, const double* xmin_host, int nbisect,
+                            double* xnew_dev, void* stream);
+
 /* RHS F(rho) of one physics (assemble_F_fast / assemble_F_fast_scalar,
  * _utils.py:453-507), in the plane-major layout. */
 int maple_rhs(maple_ctx* ctx, int physics, const double* rho_dev, double* F_dev, void* stream);

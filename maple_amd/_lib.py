@@ -59,6 +59,7 @@ SIGNATURES = {
     "maple_filter_box": (_I, [_P, _P, _P, _I, _P]),
     "maple_symmetrize": (_I, [_P, _I, _I, _P, _P, _P]),
     "maple_oc_update": (_I, [_P, _P, _P, _DP, _DP, _DP, _I, _P, _P]),
+    "maple_oc_update_noalloc": (_I, [_I, _I, _IP, _P, _P, _DP, _DP, _DP, _I, _P, _P]),
     "maple_rhs": (_I, [_P, _I, _P, _P, _P]),
     "maple_apply_operator": (_I, [_P, _I, _P, _I, _P, _P, _P]),
     "maple_get_fields": (_I, [_P, _I, _P, _P]),

@@ -277,7 +277,16 @@ __device__ __forceinline__ double oc_value(double a, double x, double c, double 
   return fmin(fmax(v, xmin), 1.0);         // torch.clamp(v, xmin, 1)
 }
 
+// a = x sqrt(max(dJ, 1e-12)) of one element: read from the prepared array, or (FROM_DJ)
+// formed from dJ on the fly, to the same bits
+template <bool FROM_DJ>
+__device__ __forceinline__ double oc_a(const double* __restrict__ a_or_dJ, double xv, long long e) {
+  if constexpr (FROM_DJ) return xv * sqrt(fmax(a_or_dJ[e], 1e-12));
+  else return a_or_dJ[e];
+}
+
 // per block: partial[blk*64 + t] = sum over the block's elements of x'(lam_t)
+template <bool FROM_DJ>
 __global__ __launch_bounds__(256) void k_oc_pass(Launch L, const OCState* __restrict__ st,
                                                  const double* __restrict__ x, const double* __restrict__ a,
                                                  double* __restrict__ partial) {
@@ -293,7 +302,7 @@ __global__ __launch_bounds__(256) void k_oc_pass(Launch L, const OCState* __rest
   for (int q = 0; q < 4; ++q) {
     const long long n = n0 + q * 256 + threadIdx.x;
     if (n < nn) {
-      const double av = a[c.off + n], xv = x[c.off + n];
+      const double xv = x[c.off + n], av = oc_a<FROM_DJ>(a, xv, c.off + n);
 #pragma unroll
       for (int t = 1; t < 64; ++t) acc[t] += oc_value(av, xv, s.c[t], move, xmin);
     }
@@ -312,7 +321,12 @@ __global__ __launch_bounds__(256) void k_oc_pass(Launch L, const OCState* __rest
 }
 
 void oc_pass(const Launch& L, const OCState* st, const double* x, const double* a, double* partial, hipStream_t s) {
-  k_oc_pass<<<L.nblocks, 256, 0, s>>>(L, st, x, a, partial);
+  k_oc_pass<false><<<L.nblocks, 256, 0, s>>>(L, st, x, a, partial);
+}
+
+void oc_pass_dj(const Launch& L, const OCState* st, const double* x, const double* dJ, double* partial,
+                hipStream_t s) {
+  k_oc_pass<true><<<L.nblocks, 256, 0, s>>>(L, st, x, dJ, partial);
 }
 
 // one block of 1024 per cell: the 63 candidate means (16 slices of blocks per
@@ -363,6 +377,7 @@ void oc_walk(const int* blk_start, const LevelCell* cells, int ncells, int depth
   k_oc_walk<<<ncells, 1024, 0, s>>>(blk_start, cells, depth, partial, st);
 }
 
+template <bool FROM_DJ>
 __global__ __launch_bounds__(256) void k_oc_final(Launch L, const OCState* __restrict__ st,
                                                   const double* __restrict__ x, const double* __restrict__ a,
                                                   double* __restrict__ xnew) {
@@ -372,9 +387,14 @@ __global__ __launch_bounds__(256) void k_oc_final(Launch L, const OCState* __res
   if (n >= (long long)c.nx * c.ny * c.nz) return;
   const OCState& s = st[ci];
   const long long e = c.off + n;
-  xnew[e] = oc_value(a[e], x[e], 1.0 / sqrt(s.last_lam), s.move, s.xmin);
+  const double xv = x[e];
+  xnew[e] = oc_value(oc_a<FROM_DJ>(a, xv, e), xv, 1.0 / sqrt(s.last_lam), s.move, s.xmin);
 }
 
 void oc_final(const Launch& L, const OCState* st, const double* x, const double* a, double* xnew, hipStream_t s) {
-  k_oc_final<<<L.nblocks, 256, 0, s>>>(L, st, x, a, xnew);
+  k_oc_final<false><<<L.nblocks, 256, 0, s>>>(L, st, x, a, xnew);
+}
+
+void oc_final_dj(const Launch& L, const OCState* st, const double* x, const double* dJ, double* xnew, hipStream_t s) {
+  k_oc_final<true><<<L.nblocks, 256, 0, s>>>(L, st, x, dJ, xnew);
 }

@@ -239,6 +239,10 @@ void oc_pass(const Launch& L, const OCState* st, const double* x, const double* 
 void oc_walk(const int* blk_start, const LevelCell* cells, int ncells, int depth, const double* partial,
              OCState* st, hipStream_t s);
 void oc_final(const Launch& L, const OCState* st, const double* x, const double* a, double* xnew, hipStream_t s);
+// the same passes forming a = x sqrt(max(dJ, 1e-12)) on the fly from dJ (no a array)
+void oc_pass_dj(const Launch& L, const OCState* st, const double* x, const double* dJ, double* partial,
+                hipStream_t s);
+void oc_final_dj(const Launch& L, const OCState* st, const double* x, const double* dJ, double* xnew, hipStream_t s);
 
 // ---- island correction of the preconditioner (islands.hip) -------------------
 #define MAPLE_ISL_MAXE 16     // elements of an island
