@@ -40,7 +40,7 @@ Beside the headline (each can be switched off):
   counted, not fatal.
 * ``config3``: one 64^3 cell in the mixed form, J = 0.4 E + 0.3 G + 0.3 kappa, 100
   OC iterations; at every iterate C_H, kappa_H and dJ/drho against a tight
-  (rtol 1e-10) FP64 solve of the same density (the FP64 path is pinned to the
+  (rtol 1e-8) FP64 solve of the same density (the FP64 path is pinned to the
   FP64 oracle by the tests), max and last error reported.
 * ``config1_gpu`` and ``cpu_baseline``: the 32^3 sphere-void cell, bulk modulus,
   3 levels, FP64, the plain method; the GPU runs 50 OC iterations, the CPU oracle
@@ -49,7 +49,7 @@ Beside the headline (each can be switched off):
   the GPU's rate over the same iterations is reported beside it.
 
 Accuracy: the last iterate's C_H and kappa of every form against a tight
-(rtol 1e-10) FP64 solve of the same densities.  ``roofline`` is measured live
+(rtol 1e-8) FP64 solve of the same densities.  ``roofline`` is measured live
 for the dominant kernel (the level-0 elastic operator of the headline form's
 preconditioner) with HIP events on the stream it runs on.
 """
@@ -160,7 +160,7 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
     CHl = np.concatenate([r[1]["CH"] for r in res])
     kl = np.concatenate([r[1]["kappa"] for r in res])
     ref = maple_amd.CellBatch([R] * ncells, nlevels=NLEVELS, device=dev)
-    CHt, kt, _ = ref.homogenize(rho, ("elastic", "thermal"), rtol=1e-10, maxit=4 * maxit, strict=False)
+    CHt, kt, _ = ref.homogenize(rho, ("elastic", "thermal"), rtol=1e-8, maxit=4 * maxit, strict=False)
     CH6 = torch.from_numpy(CHl).to(dev)
     k6 = torch.from_numpy(kl).to(dev)
     err = 0.0
@@ -232,7 +232,13 @@ def run_config4(steps: int, precision: str = "mixed", maxit: int = 3000):
 
 
 def run_config3(steps: int):
-    """One 64^3 cell, mixed form, tri-objective; per-iterate error against a tight FP64 solve."""
+    """One 64^3 cell, mixed form, tri-objective; per-iterate error against a tight FP64 solve.
+
+    Reported per iterate: C_H, kappa_H, dJ/drho and dJ/dx (the filtered sensitivity the
+    OC step consumes) of the mixed run against the tight FP64 solve of the same
+    density, and, as the yardstick, the same errors of the FP64 form at the mixed
+    run's own tolerance (rtol 1e-6): what rtol alone leaves in the sensitivities of
+    the 1e-9 voids, whose fields the energy barely constrains."""
     import numpy as np
     import torch
 
@@ -242,31 +248,47 @@ def run_config3(steps: int):
     w = {"stiffness": 0.4, "shear": 0.3, "thermal": 0.3}
     cm = maple_amd.CellBatch([R], nlevels=NLEVELS, device=dev, precision="mixed", warm_start=True)
     cf = maple_amd.CellBatch([R], nlevels=NLEVELS, device=dev, warm_start=True)
+    c6 = maple_amd.CellBatch([R], nlevels=NLEVELS, device=dev, warm_start=True)
     drv = maple_amd.OCDriver(cm, w, 0.25, move=0.2, xmin=1e-3, filter_k=3, rtol=1e-6, maxit=5000)
     x = torch.from_numpy(maple_amd.x0_uniform_noise(R, seed=0)).to(dev)
-    eC, ek, ed = [], [], []
+    err = {k: [] for k in ("ch", "kappa", "dJdrho", "dJdx", "fp64_rtol1e-6_dJdrho", "fp64_rtol1e-6_dJdx")}
     t_mixed = 0.0
+
+    def rel(a, b):
+        return float((a - b).abs().max() / b.abs().max())
+
     for _ in range(steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        x_used = x.clamp(1e-3, 1.0)
         x, info = drv.step(x)
         torch.cuda.synchronize()
         t_mixed += time.perf_counter() - t0
-        CHf, kf, _ = cf.homogenize(info["rho"], rtol=1e-10, maxit=20000, strict=False)
-        df = cf.sensitivity(torch.from_numpy(info["G6"]).to(dev), torch.from_numpy(info["G3"]).to(dev))
+        G6, G3 = torch.from_numpy(info["G6"]).to(dev), torch.from_numpy(info["G3"]).to(dev)
+        dxr = 3.0 * (1.0 - 1e-9) * x_used ** 2
+        CHf, kf, _ = cf.homogenize(info["rho"], rtol=1e-8, maxit=20000, strict=False)
+        df = cf.sensitivity(G6, G3)
+        dxf = cf.filter(df * dxr, 3)
+        c6.homogenize(info["rho"], rtol=1e-6, maxit=20000, strict=False)
+        d6 = c6.sensitivity(G6, G3)
         CHf, kf = CHf[0].cpu().numpy(), kf[0].cpu().numpy()
-        eC.append(float(np.linalg.norm(info["CH"][0] - CHf) / np.linalg.norm(CHf)))
-        ek.append(float(np.linalg.norm(info["kappa"][0] - kf) / np.linalg.norm(kf)))
-        ed.append(float((info["dJdrho"] - df).abs().max() / df.abs().max()))
-    cm.close()
-    cf.close()
-    return {"steps": steps, "oc_iters_per_s_mixed": steps / t_mixed,
-            "ch_rel_err_max": max(eC), "kappa_rel_err_max": max(ek), "dJdrho_rel_err_max": max(ed),
-            "ch_rel_err_last": eC[-1], "kappa_rel_err_last": ek[-1], "dJdrho_rel_err_last": ed[-1],
-            "per_iteration": {"ch": [float(f"{v:.3g}") for v in eC], "kappa": [float(f"{v:.3g}") for v in ek],
-                              "dJdrho": [float(f"{v:.3g}") for v in ed]},
-            "reference_solve": "library FP64 path at rtol 1e-10 on the same density (pinned to the FP64 oracle "
-                               "by tests/test_gpu_configs.py at 64^3)"}
+        err["ch"].append(float(np.linalg.norm(info["CH"][0] - CHf) / np.linalg.norm(CHf)))
+        err["kappa"].append(float(np.linalg.norm(info["kappa"][0] - kf) / np.linalg.norm(kf)))
+        err["dJdrho"].append(rel(info["dJdrho"], df))
+        err["dJdx"].append(rel(info["dJdx"], dxf))
+        err["fp64_rtol1e-6_dJdrho"].append(rel(d6, df))
+        err["fp64_rtol1e-6_dJdx"].append(rel(c6.filter(d6 * dxr, 3), dxf))
+    for c in (cm, cf, c6):
+        c.close()
+    out = {"steps": steps, "oc_iters_per_s_mixed": steps / t_mixed,
+           "error": "relative: C_H, kappa_H Frobenius; dJ/drho, dJ/dx max-norm over the largest entry",
+           "reference_solve": "library FP64 path at rtol 1e-8 on the same density (pinned to the FP64 oracle "
+                              "by tests/test_gpu_configs.py at 64^3)"}
+    for k, v in err.items():
+        out[f"{k}_rel_err_max"] = max(v)
+        out[f"{k}_rel_err_last"] = v[-1]
+    out["per_iteration"] = {k: [float(f"{e:.3g}") for e in v] for k, v in err.items()}
+    return out
 
 
 def _cpu_model():
@@ -445,8 +467,10 @@ def main():
               file=sys.stderr, flush=True)
     if not args.no_config3:
         line["config3"] = run_config3(TRAJ)
-        print(f"# config3: max C_H err {line['config3']['ch_rel_err_max']:.2e}, dJ/drho "
-              f"{line['config3']['dJdrho_rel_err_max']:.2e}", file=sys.stderr, flush=True)
+        c3 = line["config3"]
+        print(f"# config3: max C_H err {c3['ch_rel_err_max']:.2e}, dJ/drho {c3['dJdrho_rel_err_max']:.2e} "
+              f"(FP64 at rtol 1e-6: {c3['fp64_rtol1e-6_dJdrho_rel_err_max']:.2e}), dJ/dx {c3['dJdx_rel_err_max']:.2e}",
+              file=sys.stderr, flush=True)
     window = (3, 2 + args.cpu_iters)
     if not args.no_config1:
         line["config1_gpu"] = run_config1_gpu(50, window)

@@ -229,6 +229,45 @@ int maple_oc_update_noalloc(int device, int ncells, const int* dims_host, const 
                             const double* volfrac_host, const double* move_host, const double* xmin_host, int nbisect,
                             double* xnew_dev, void* stream);
 
+/* ---- additive-manufacturing penalties ---------------------------------------
+ * Differentiable penalties on the SIMP density of every cell (cubic cells, grid
+ * (x, y, z) with the build direction +z), each with its gradient dP/drho, for
+ * J_total = J - lambda_print P_print - lambda_cavity P_cavity (main_IHD_v3.py:686).
+ * The gradients are the subgradients the reference's autograd takes through
+ * max-pooling (first maximum of a window in raster order), minimum (ties halved)
+ * and relu.  P_dev: ncells doubles; dP_dev: Etot doubles or NULL (value only). */
+
+/* printability_penalty_overhang (printability.py:91-134): a layer is supported by
+ * the dilated support of the layer below (lateral radius ceil(tan(90 - deg)),
+ * at most 7; periodic_xy wraps the layer); cells up to 64^3. */
+int maple_penalty_overhang(maple_ctx* ctx, const double* rho_dev, double overhang_deg, int periodic_xy, double* P_dev,
+                           double* dP_dev, void* stream);
+
+#define MAPLE_SEED_ALL 0
+#define MAPLE_SEED_X 1
+#define MAPLE_SEED_Y 2
+#define MAPLE_SEED_Z 3
+#define MAPLE_SEED_XY 4     /* the +/-z faces (drainage along z), as the reference's "xy" */
+typedef struct {
+  double void_thresh;       /* soft void indicator sigmoid(beta (void_thresh - rho)); 0.5 */
+  double beta;              /* 20 */
+  int min_channel_rad;      /* soft opening radius before the flood fill, 0..3 (0: none) */
+  int rad;                  /* flood-fill neighbourhood radius, 0 or 1 */
+  int iters;                /* flood-fill steps, 0: the supercell's largest side */
+  int periodic;             /* one-cell form: periodic padding (1) or outside = 0 (0) */
+  int seed_faces;           /* MAPLE_SEED_* */
+  int tiles[3];             /* all 1: cavity_penalty_closed_void (printability.py:139-223);
+                               else cavity_penalty_periodic_array (printability.py:229-323) on
+                               the unit cell tiled tiles[0] x tiles[1] x tiles[2], outside = 0 */
+  int focus_center;         /* periodic array with every tile count >= 3: count the centre cell only */
+} maple_cavity_opts;
+
+/* Configure the trapped-void penalty and allocate its buffers (per voxel of every
+ * cell's supercell: 9 doubles, 2 shorts and one byte per flood-fill step). */
+int maple_set_cavity(maple_ctx* ctx, const maple_cavity_opts* opts);
+/* The trapped-void penalty as configured by maple_set_cavity. */
+int maple_penalty_cavity(maple_ctx* ctx, const double* rho_dev, double* P_dev, double* dP_dev, void* stream);
+
 /* RHS F(rho) of one physics (assemble_F_fast / assemble_F_fast_scalar,
  * _utils.py:453-507), in the plane-major layout. */
 int maple_rhs(maple_ctx* ctx, int physics, const double* rho_dev, double* F_dev, void* stream);

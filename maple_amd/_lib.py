@@ -60,6 +60,9 @@ SIGNATURES = {
     "maple_symmetrize": (_I, [_P, _I, _I, _P, _P, _P]),
     "maple_oc_update": (_I, [_P, _P, _P, _DP, _DP, _DP, _I, _P, _P]),
     "maple_oc_update_noalloc": (_I, [_I, _I, _IP, _P, _P, _DP, _DP, _DP, _I, _P, _P]),
+    "maple_penalty_overhang": (_I, [_P, _P, _D, _I, _P, _P, _P]),
+    "maple_set_cavity": (_I, [_P, _P]),
+    "maple_penalty_cavity": (_I, [_P, _P, _P, _P, _P]),
     "maple_rhs": (_I, [_P, _I, _P, _P, _P]),
     "maple_apply_operator": (_I, [_P, _I, _P, _I, _P, _P, _P]),
     "maple_get_fields": (_I, [_P, _I, _P, _P]),
@@ -89,6 +92,16 @@ SIGNATURES = {
     "maple_unet_loss_grad": (_I, [_P, _P, _P, _I, _DP, _P, _P]),
     "maple_unet_train_step": (_I, [_P, _P, _P, _I, C.c_float, _DP, _P]),
 }
+
+SEED_FACES = {"all": 0, "x": 1, "y": 2, "z": 3, "xy": 4}
+
+
+class CavityOpts(C.Structure):
+    """maple_cavity_opts of maple_hip.h."""
+    _fields_ = [("void_thresh", C.c_double), ("beta", C.c_double), ("min_channel_rad", C.c_int), ("rad", C.c_int),
+                ("iters", C.c_int), ("periodic", C.c_int), ("seed_faces", C.c_int), ("tiles", C.c_int * 3),
+                ("focus_center", C.c_int)]
+
 
 _lib = None
 

@@ -281,6 +281,39 @@ class CellBatch:
                                        C.c_void_p(fine.data_ptr()), self._stream()))
         return fine
 
+    # -- additive-manufacturing penalties ---------------------------------------
+    def penalty_overhang(self, rho: torch.Tensor, overhang_deg: float = 45.0, periodic_xy: bool = True,
+                         grad: bool = True):
+        """printability_penalty_overhang of every cell: (P (ncells,), dP/drho (Etot,) or None)."""
+        rho = self._vec(rho, self.nelem, "rho")
+        P = self._empty(self.ncells)
+        dP = self._empty(self.nelem) if grad else None
+        L.check(self.lib.maple_penalty_overhang(self._h, C.c_void_p(rho.data_ptr()), float(overhang_deg),
+                                                int(bool(periodic_xy)), C.c_void_p(P.data_ptr()),
+                                                C.c_void_p(dP.data_ptr()) if dP is not None else None,
+                                                self._stream()))
+        return P, dP
+
+    def set_cavity(self, void_thresh: float = 0.5, beta: float = 20.0, min_channel_rad: int = 0, rad: int = 1,
+                   iters: int = 0, periodic_xyz: bool = True, seed_faces: str = "all", tiles=(1, 1, 1),
+                   focus_center: bool = True):
+        """Configure (and allocate) the trapped-void penalty: tiles all 1 is
+        cavity_penalty_closed_void, otherwise cavity_penalty_periodic_array on the tiled cell."""
+        t = (int(tiles),) * 3 if isinstance(tiles, (int, np.integer)) else tuple(int(v) for v in tiles)
+        o = L.CavityOpts(float(void_thresh), float(beta), int(min_channel_rad), int(rad), int(iters),
+                         int(bool(periodic_xyz)), L.SEED_FACES[str(seed_faces).lower()], (C.c_int * 3)(*t),
+                         int(bool(focus_center)))
+        L.check(self.lib.maple_set_cavity(self._h, C.byref(o)))
+
+    def penalty_cavity(self, rho: torch.Tensor, grad: bool = True):
+        """The configured trapped-void penalty: (P (ncells,), dP/drho (Etot,) or None)."""
+        rho = self._vec(rho, self.nelem, "rho")
+        P = self._empty(self.ncells)
+        dP = self._empty(self.nelem) if grad else None
+        L.check(self.lib.maple_penalty_cavity(self._h, C.c_void_p(rho.data_ptr()), C.c_void_p(P.data_ptr()),
+                                              C.c_void_p(dP.data_ptr()) if dP is not None else None, self._stream()))
+        return P, dP
+
     # -- multigrid internals -------------------------------------------------
     def _prec(self, precision: str) -> int:
         return {"fp64": L.MAPLE_PREC_FP64, "mixed": L.MAPLE_PREC_MIXED}[precision]

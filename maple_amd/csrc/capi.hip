@@ -151,6 +151,11 @@ struct maple_ctx {
   int* isl_count_h = nullptr;                   // pinned
   Island* isl = nullptr;
   int nisl = 0;                                 // islands of the current density
+  // AM penalties (penalty.hip)
+  double* oh_tape = nullptr;                    // overhang: the support layers (Etot)
+  CavityBuffers cav;
+  bool cav_ready = false;
+  std::vector<void*> cav_allocs;                // freed when the configuration changes
   bool timing = false;
   Timer timers[3];   // 0: FP64 elastic level-0 operator, 1: FP64 thermal, 2: FP32 elastic
   unsigned long long* work_d = nullptr;   // per timer: elements x load cases of the timed launches
@@ -676,6 +681,7 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   const bool flex = c->flexible < 0 ? c->prec != MAPLE_PREC_FP64 : c->flexible != 0;
   bool use64 = c->prec == MAPLE_PREC_FP64;
   bool want64 = false;   // a CG restart was seen: finish on the FP64 V-cycle
+  g_err.clear();
   // second partial-sum area of the solve (the mixed update's fused flexible dot); the
   // third (dot_rz_mean's sums of z) follows it
   double* partial2 = P.partial + (size_t)c->lv[0].node.nblocks * MAPLE_MAX_CASES;
@@ -762,10 +768,16 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
                                         std::to_string(cs.iters) + " iterations)");
   }
   for (size_t i = 0; i < P.hsc.size(); ++i)
-    if (P.hsc[i].brk)
+    if (P.hsc[i].brk == 1)
       return fail(MAPLE_EBREAKDOWN, "CG breakdown (p^T K p <= 0) in cell " + std::to_string(i / P.nc) + ", load case " +
                                         std::to_string(i % P.nc) + " after " + std::to_string(P.hsc[i].iters) +
                                         " iterations; its fields are not converged");
+  for (size_t i = 0; i < P.hsc.size(); ++i)
+    if (P.hsc[i].brk == 2)
+      return fail(MAPLE_ENOCONV, "CG stagnated before rtol in cell " + std::to_string(i / P.nc) + ", load case " +
+                                     std::to_string(i % P.nc) + " (no new residual minimum after iteration " +
+                                     std::to_string(P.hsc[i].it_min) + " of " + std::to_string(P.hsc[i].iters) +
+                                     "): rtol is below the accuracy this operator allows");
   return all ? 0 : MAPLE_ENOCONV;
 }
 
@@ -860,7 +872,8 @@ int solve_physics(maple_ctx* c, int p, const double* rho, double rtol, int maxit
   HIPCHK(hipGetLastError());
   int status = solve_cases(c, P, rho, rtol, maxit, s);
   if (status == MAPLE_ENOCONV) {
-    g_err = std::string(p == 0 ? "elastic" : "thermal") + " CG reached maxit before rtol";
+    if (g_err.rfind("CG stagnated", 0) == 0) g_err = std::string(p == 0 ? "elastic: " : "thermal: ") + g_err;
+    else g_err = std::string(p == 0 ? "elastic" : "thermal") + " CG reached maxit before rtol";
   } else if (status == MAPLE_EBREAKDOWN) {
     g_err = std::string(p == 0 ? "elastic: " : "thermal: ") + g_err;
   } else if (status) {
@@ -1018,6 +1031,7 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
     if ((rc = upload_phys_constants(c, c->ph[p]))) return bail(rc);
   if ((rc = dalloc(c, &c->ocst, ncells))) return bail(rc);
   if ((rc = dalloc(c, &c->oc_a, (size_t)c->lv[0].Ntot))) return bail(rc);
+  if ((rc = dalloc(c, &c->oh_tape, (size_t)c->lv[0].Ntot))) return bail(rc);
   void* hp = nullptr;
   if (hipHostMalloc(&hp, sizeof(OCState) * ncells, hipHostMallocDefault) != hipSuccess)
     return bail(fail(MAPLE_ENOMEM, "pinned host allocation failed"));
@@ -1045,6 +1059,7 @@ int maple_ctx_destroy(maple_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
   for (void* p : c->allocs) (void)hipFree(p);
+  for (void* p : c->cav_allocs) (void)hipFree(p);
   for (void* p : c->host_allocs) (void)hipHostFree(p);
   for (Phys& P : c->ph)
     for (int k = 0; k < 2; ++k)
@@ -1580,6 +1595,123 @@ int maple_mg_coarsest_inverse(maple_ctx* c, int physics, int cell, double* Ainv,
 int maple_mg_coarsest_order(const maple_ctx* c, int physics, int cell) {
   if (!c || cell < 0 || cell >= c->ncells || (physics != MAPLE_ELASTIC && physics != MAPLE_THERMAL)) return 0;
   return c->ph[physics == MAPLE_ELASTIC ? 0 : 1].hdc[cell].n;
+}
+
+}  // extern "C"
+
+// ---- additive-manufacturing penalties ---------------------------------------------
+
+extern "C" {
+
+int maple_penalty_overhang(maple_ctx* c, const double* rho, double deg, int periodic_xy, double* P, double* dP,
+                           void* stream) {
+  TRY(check_ctx(c));
+  if (!rho || !P) return fail(MAPLE_EINVAL, "null argument");
+  const double od = std::min(89.0, std::max(1.0, deg));
+  const int rad = std::max(1, (int)std::ceil(std::tan((90.0 - od) * M_PI / 180.0)));
+  if (rad > 7) return fail(MAPLE_EINVAL, "overhang angle too shallow: support radius above 7 voxels");
+  int rmax = 0;
+  for (int i = 0; i < c->ncells; ++i) rmax = std::max(rmax, c->dims[3 * i]);
+  if (rmax > 64) return fail(MAPLE_EINVAL, "the overhang penalty takes cells up to 64^3");
+  if (periodic_xy && rmax > 0)
+    for (int i = 0; i < c->ncells; ++i)
+      if (c->dims[3 * i] < 2 * rad + 1) return fail(MAPLE_EINVAL, "cell smaller than the support window");
+  overhang_penalty(c->lv[0].d, c->ncells, rmax, rho, rad, periodic_xy ? 1 : 0, c->oh_tape, P, dP, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int maple_set_cavity(maple_ctx* c, const maple_cavity_opts* o) {
+  TRY(check_ctx(c));
+  if (!o) return fail(MAPLE_EINVAL, "null options");
+  if (o->min_channel_rad < 0 || o->min_channel_rad > 3) return fail(MAPLE_EINVAL, "min_channel_rad must lie in [0, 3]");
+  if (o->rad < 0 || o->rad > 1) return fail(MAPLE_EINVAL, "the flood-fill radius must be 0 or 1");
+  if (o->iters < 0 || o->iters > 1024) return fail(MAPLE_EINVAL, "iters must lie in [0, 1024]");
+  if (o->seed_faces < MAPLE_SEED_ALL || o->seed_faces > MAPLE_SEED_XY) return fail(MAPLE_EINVAL, "unknown seed faces");
+  if (!(o->beta > 0.0)) return fail(MAPLE_EINVAL, "beta must be positive");
+  int t[3];
+  for (int d = 0; d < 3; ++d) {
+    t[d] = std::max(1, o->tiles[d]);
+    if (t[d] > 8) return fail(MAPLE_EINVAL, "at most 8 tiles per axis");
+  }
+  const bool array = t[0] > 1 || t[1] > 1 || t[2] > 1;
+  for (void* p : c->cav_allocs) (void)hipFree(p);
+  c->cav_allocs.clear();
+  c->cav_ready = false;
+  CavityBuffers& B = c->cav;
+  B = CavityBuffers();
+  B.beta = o->beta;
+  B.thr = o->void_thresh;
+  B.mcr = o->min_channel_rad;
+  B.rad = o->rad;
+  B.periodic = array ? 0 : (o->periodic ? 1 : 0);
+  B.faces = o->seed_faces;
+  B.focus = array && o->focus_center && t[0] >= 3 && t[1] >= 3 && t[2] >= 3;
+  std::vector<LevelCell> sc(c->ncells);
+  std::vector<int> blk(c->ncells + 1, 0), reg(3 * c->ncells);
+  long long off = 0;
+  int side = 0;
+  for (int i = 0; i < c->ncells; ++i) {
+    LevelCell s;
+    s.nx = t[0] * c->dims[3 * i];
+    s.ny = t[1] * c->dims[3 * i + 1];
+    s.nz = t[2] * c->dims[3 * i + 2];
+    s.id = i;
+    s.off = off;
+    sc[i] = s;
+    const long long n = (long long)s.nx * s.ny * s.nz;
+    off += n;
+    blk[i + 1] = blk[i] + (int)((n + 255) / 256);
+    for (int d = 0; d < 3; ++d) reg[3 * i + d] = c->dims[3 * i + d];
+    side = std::max(side, std::max(s.nx, std::max(s.ny, s.nz)));
+    if (B.periodic && std::min(s.nx, std::min(s.ny, s.nz)) < 2 * std::max(B.mcr, B.rad) + 1)
+      return fail(MAPLE_EINVAL, "cell smaller than the pooling window");
+  }
+  B.nsteps = o->iters > 0 ? o->iters : side;
+  const long long G = off;
+  auto al = [&](auto** p, size_t n) -> int {
+    using T = typename std::remove_pointer<typename std::remove_reference<decltype(*p)>::type>::type;
+    *p = nullptr;
+    if (hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(MAPLE_ENOMEM, "hipMalloc of the cavity buffers failed (" + std::to_string(n * sizeof(T)) + " bytes)");
+    }
+    c->cav_allocs.push_back((void*)*p);
+    return 0;
+  };
+  LevelCell* dsc = nullptr;
+  int* dblk = nullptr;
+  TRY(al(&dsc, c->ncells));
+  TRY(al(&dblk, c->ncells + 1));
+  TRY(al(&B.region, 3 * c->ncells));
+  HIPCHK(hipMemcpy(dsc, sc.data(), sizeof(LevelCell) * sc.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dblk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(B.region, reg.data(), sizeof(int) * reg.size(), hipMemcpyHostToDevice));
+  B.L.cells = dsc;
+  B.L.blk_start = dblk;
+  B.L.ncells = c->ncells;
+  B.L.nblocks = blk.back();
+  B.L.Ntot = G;
+  for (double** p : {&B.voidv, &B.reach[0], &B.reach[1], &B.g[0], &B.g[1], &B.gvc, &B.gvoid}) TRY(al(p, (size_t)G));
+  if (B.mcr > 0) {
+    TRY(al(&B.vc, (size_t)G));
+    TRY(al(&B.er, (size_t)G));
+    TRY(al(&B.argE, (size_t)G));
+    TRY(al(&B.argD, (size_t)G));
+  }
+  TRY(al(&B.partial, (size_t)B.L.nblocks));
+  if (B.rad > 0) TRY(al(&B.tape, (size_t)G * B.nsteps));
+  c->cav_ready = true;
+  return 0;
+}
+
+int maple_penalty_cavity(maple_ctx* c, const double* rho, double* P, double* dP, void* stream) {
+  TRY(check_ctx(c));
+  if (!c->cav_ready) return fail(MAPLE_EINVAL, "configure the cavity penalty first (maple_set_cavity)");
+  if (!rho || !P) return fail(MAPLE_EINVAL, "null argument");
+  TRY(cavity_penalty(c->cav, c->lv[0].d, c->lv[0].node, rho, P, dP, (hipStream_t)stream));
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 }  // extern "C"

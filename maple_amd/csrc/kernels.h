@@ -123,12 +123,16 @@ struct CaseScalars {   // one per (cell, case)
   double alpha, beta;
   int done, iters;
   int src;             // start of the last solve: 0 zero, 1 previous fields, 2 supplied start
-  int brk;             // CG breakdown: p^T K p <= 0 (or NaN) on MAPLE_CG_RESTARTS directions in a row
+  int brk;             // stopped early: 1 breakdown (p^T K p <= 0 on MAPLE_CG_RESTARTS directions in
+                       // a row), 2 stagnation (no new residual minimum in MAPLE_CG_STALL iterations,
+                       // or the residual 1e6 times above its minimum)
   int nrst;            // consecutive restarts (p^T K p <= 0): the next direction is p = z
-  int pad_;
+  int it_min;          // iteration of the smallest residual so far
+  double rr_min;       // smallest squared residual so far
   double zmean[3];     // per component: mean of the preconditioned residual over the cell's nodes
 };
 #define MAPLE_CG_RESTARTS 3
+#define MAPLE_CG_STALL 400
 // Block CG state of one cell (bcg.hip): Gram matrices of the current step (two
 // NC x NC, row-major, at 0 and 36), the step matrices, the Cholesky factor of
 // P^T K P, the orthonormalising transform and the active directions.
@@ -243,6 +247,26 @@ void oc_final(const Launch& L, const OCState* st, const double* x, const double*
 void oc_pass_dj(const Launch& L, const OCState* st, const double* x, const double* dJ, double* partial,
                 hipStream_t s);
 void oc_final_dj(const Launch& L, const OCState* st, const double* x, const double* dJ, double* xnew, hipStream_t s);
+
+// ---- AM penalties (penalty.hip) ----------------------------------------------------
+// Overhang penalty of every cell (cubic, r <= 64, rad <= 7): P[cell]; dP/drho into grad
+// (Etot, may be null); tape: Etot doubles (the support layers, kept for the adjoint).
+void overhang_penalty(const LevelCell* cells, int ncells, int rmax, const double* rho, int rad, int periodic,
+                      double* tape, double* P, double* grad, hipStream_t s);
+// Trapped-void penalty: buffers over the cells' supercells (unit cell tiled tx x ty x tz)
+struct CavityBuffers {
+  Launch L;                 // supercell voxels, 256 per block (L.cells: one supercell per cell)
+  double beta = 20.0, thr = 0.5;
+  int mcr = 0, rad = 1, nsteps = 1, periodic = 1, faces = 0, focus = 0;
+  int* region = nullptr;    // 3 per cell: the unit cell's sides (the counted centre tile)
+  double *voidv = nullptr, *vc = nullptr, *er = nullptr, *reach[2] = {nullptr, nullptr};
+  double *g[2] = {nullptr, nullptr}, *gvc = nullptr, *gvoid = nullptr, *partial = nullptr;
+  short *argE = nullptr, *argD = nullptr;
+  unsigned char* tape = nullptr;   // nsteps x voxels
+};
+// P[cell] and (grad not null) dP/drho over the unit cells; Lu: the level-0 node launch
+int cavity_penalty(const CavityBuffers& B, const LevelCell* uc, const Launch& Lu, const double* rho, double* P,
+                   double* grad, hipStream_t s);
 
 // ---- island correction of the preconditioner (islands.hip) -------------------
 #define MAPLE_ISL_MAXE 16     // elements of an island

@@ -419,6 +419,8 @@ __global__ void k_pcg_init(int n, int ncase, double rtol, const double* __restri
   s.iters = 0;
   s.brk = 0;
   s.nrst = 0;
+  s.rr_min = s.rr;
+  s.it_min = 0;
   s.alpha = s.beta = 0.0;
   s.rz = s.rz_new;
   done_flags[t] = s.done;
@@ -469,7 +471,17 @@ __global__ void k_pcg_check(int n, CaseScalars* __restrict__ sc, int* __restrict
   CaseScalars& s = sc[t];
   if (!s.done) {
     s.iters += 1;
-    if (s.rr <= s.tgt2 || !(s.rr == s.rr)) s.done = 1;
+    if (s.rr <= s.tgt2 || !(s.rr == s.rr)) {
+      s.done = 1;
+    } else if (s.rr < s.rr_min) {
+      s.rr_min = s.rr;
+      s.it_min = s.iters;
+    } else if (s.iters - s.it_min >= MAPLE_CG_STALL || s.rr > 1e12 * s.rr_min) {
+      // below the attainable accuracy (the recursive residual no longer falls) or
+      // diverging: stop with the fields reached, before rounding turns them into NaN
+      s.done = 1;
+      s.brk = 2;
+    }
   }
   done_flags[t] = s.done;
 }

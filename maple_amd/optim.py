@@ -172,13 +172,15 @@ class OCDriver:
     def __init__(self, cells: CellBatch, objectives: Sequence, volfrac: Sequence[float], penal: float = 3.0,
                  rho_min: float = 1e-9, move: float = 0.2, xmin: float = 1e-3, filter_k: int = 3,
                  rtol: float = 1e-6, maxit: int = 1000, physics: Optional[Sequence[str]] = None, start=None,
-                 strict: bool = True):
+                 strict: bool = True, penalties=None):
         """physics: solve these physics whatever the objectives need (None: what they need).
         start: callable(rho, physics) that supplies start candidates before each
         homogenization (e.g. maple_amd.learned.LearnedStart).
         strict: raise when a solve stops before rtol (maxit or CG breakdown); with
         strict=False the step goes on with the fields reached and reports the (cell,
-        case) pairs that stopped early in info["unconverged"]."""
+        case) pairs that stopped early in info["unconverged"].
+        penalties: a maple_amd.driver.Penalties of this batch (AM overhang and trapped-void
+        terms, J_total = J - lambda P, main_IHD_v3.py:650-688)."""
         self.cells = cells
         n = cells.ncells
         objs = list(objectives) if not isinstance(objectives, (str, dict)) else [objectives] * n
@@ -197,6 +199,7 @@ class OCDriver:
                 raise ValueError("the objectives need a physics that is not solved")
         self.start = start
         self.strict = bool(strict)
+        self.penalties = penalties if (penalties is not None and penalties.active) else None
 
     def design_to_rho(self, x: torch.Tensor) -> torch.Tensor:
         xu = x.clamp(self.xmin, 1.0)
@@ -222,11 +225,18 @@ class OCDriver:
         dev = self.cells.device
         dJdrho = self.cells.sensitivity(torch.from_numpy(G6).to(dev) if self.need_el else None,
                                         torch.from_numpy(G3).to(dev) if self.need_th else None)
+        Pp = Pc = np.zeros(n)
+        J_total = J.copy()
+        if self.penalties is not None:
+            Pp, Pc, gpen = self.penalties(rho)
+            J_total = J - self.penalties.lp * Pp - self.penalties.lc * Pc
+            dJdrho = dJdrho - gpen
         dJdx = dJdrho * (self.penal * (1.0 - self.rho_min)) * xu ** (self.penal - 1.0)
         if self.filter_k > 1:
             dJdx = self.cells.filter(dJdx, self.filter_k)
         xnew = self.cells.oc_update(x, dJdx, self.volfrac, self.move, self.xmin)
         cols = [k for p, ks in (("elastic", range(6)), ("thermal", range(6, 9))) if p in self.physics for k in ks]
         unconv = int((iters[:, cols] >= self.maxit).sum()) if status else 0
-        return xnew, {"J": J, "CH": CHh, "kappa": kh, "iters": iters.copy(), "rho": rho, "dJdx": dJdx,
+        return xnew, {"J": J, "J_total": J_total, "P_print": Pp, "P_cavity": Pc, "CH": CHh, "kappa": kh,
+                      "iters": iters.copy(), "rho": rho, "dJdx": dJdx,
                       "dJdrho": dJdrho, "G6": G6, "G3": G3, "status": int(status), "unconverged": unconv}

@@ -87,12 +87,19 @@ def run(r, x0, weights, iters, volfrac=0.25, nlevels=3, rtol=1e-6, move=0.2, xmi
 
 
 def run_driver(r, x0, weights, iters, volfrac, symmetry="none", nlevels=3, rtol=1e-10, move=0.2, xmin=1e-3,
-               p=3.0, rho_min=1e-9, filter_k=3, tol_x=1e-3, tol_J=1e-4, patience=10, min_iters=10):
+               p=3.0, rho_min=1e-9, filter_k=3, tol_x=1e-3, tol_J=1e-4, patience=10, min_iters=10,
+               lambda_print=0.0, overhang_deg=45.0, lambda_cavity=0.0, cavity=None):
     """The loop of optimize_inverse_homogenization (main_IHD_v3.py:603-758) without
-    penalties or meshes: symmetry projection of x (and, by the chain rule, of dJ/dx),
-    filter, OC, early stop.  Returns (x, rows) with rows
-    (iter, J, mean_x_used, mean_rho, dx_max, dx_mean, converged)."""
+    meshes: symmetry projection of x (and, by the chain rule, of dJ/dx), the AM
+    penalties J_total = J - lambda_print P_print - lambda_cavity P_cavity
+    (main_IHD_v3.py:650-688; cavity: keyword arguments of printability.closed_void, or
+    of periodic_array when they hold "tiles_xyz"), filter, OC, early stop.  Returns
+    (x, rows) with rows (iter, J_total, mean_x_used, mean_rho, dx_max, dx_mean,
+    converged, J_raw, P_print, P_cavity)."""
+    from . import printability as pr
     from .symmetry import parse_symmetry, symmetrize
+    cav = dict(cavity or {})
+    cav_fn = pr.periodic_array if "tiles_xyz" in cav else pr.closed_void
     kind, axes = parse_symmetry(symmetry)
     S = (lambda v: symmetrize(v, r, kind, axes)) if kind != "none" else (lambda v: np.asarray(v, dtype=np.float64))
     cs = CellSolver((r, r, r), nlevels=nlevels, rtol=rtol)
@@ -117,6 +124,15 @@ def run_driver(r, x0, weights, iters, volfrac, symmetry="none", nlevels=3, rtol=
             dJdrho += np.einsum("ij,eij->e", G6, S6)
         if need_k:
             dJdrho += np.einsum("ij,eij->e", G3, S3)
+        Pp = Pc = 0.0
+        if lambda_print > 0.0:
+            Pp, g = pr.overhang(rho, r, overhang_deg, True)
+            dJdrho = dJdrho - lambda_print * g
+        if lambda_cavity > 0.0:
+            Pc, g = cav_fn(rho, r, **cav)
+            dJdrho = dJdrho - lambda_cavity * g
+        J_raw = J
+        J = J - lambda_print * Pp - lambda_cavity * Pc
         dJdx = S(dJdrho * simp_grad(xu, p, rho_min))
         if filter_k and filter_k > 1:
             dJdx = box_filter(dJdx, cs.dims, filter_k)
@@ -127,7 +143,7 @@ def run_driver(r, x0, weights, iters, volfrac, symmetry="none", nlevels=3, rtol=
         stall = stall + 1 if (it >= min_iters and dx.max() < tol_x and dJ_rel < tol_J) else 0
         x = x_next
         J_prev = J
-        rows.append((it + 1, J, xu.mean(), rho.mean(), dx.max(), dx.mean(), stall >= patience))
+        rows.append((it + 1, J, xu.mean(), rho.mean(), dx.max(), dx.mean(), stall >= patience, J_raw, Pp, Pc))
         if stall >= patience:
             break
     return x, rows

@@ -78,7 +78,40 @@ def test_driver_early_stop(tmp_path):
         ihd, batch, res=r, objective="bulk", init_mode="rho", volfrac=0.3, penal=3.0, rho_min=1e-9, move=0.2,
         xmin=1e-3, iters=50, seed=0, symmetry="cubic", filter_k=3, output_dir=str(tmp_path), verbose=False)
     assert res.converged and res.iters_run == 20
-    with pytest.raises(NotImplementedError):
-        optimize_inverse_homogenization(
-            ihd, batch, res=r, objective="bulk", init_mode="rho", volfrac=0.3, penal=3.0, rho_min=1e-9, move=0.2,
-            xmin=1e-3, iters=2, seed=0, symmetry="none", lambda_print=0.9, output_dir=str(tmp_path), verbose=False)
+
+
+@pytest.mark.parametrize("tiles", [1, 2])
+def test_driver_with_penalties_matches_oracle(tmp_path, tiles):
+    """Printability and cavity penalties at lambda = 0.9 (the experiment scripts'
+    weights, experiments/biobjective/run_weight_sweep.sh:16-20) composed into the
+    trajectory: J_total, J_raw, P_print and P_cavity of every iteration against the
+    oracle's loop with its own penalty restatement."""
+    import maple_amd
+    from maple_amd.driver import optimize_inverse_homogenization
+    r, iters = 16, 5
+    print(f"[driver penalties] tiles={tiles}", flush=True)
+    x0 = maple_amd.x0_sphere_void(r, seed=4)
+    rho0 = 1e-9 + (1 - 1e-9) * x0 ** 3
+    ihd = maple_amd.IHD(maple_amd.make_cfg(r), DEV, rtol=1e-10, maxit=4000, nlevels=3)
+    batch = maple_amd.build_unitcell_batch(DEV, r, torch.from_numpy(rho0.reshape(r, r, r)).to(DEV))
+    res = optimize_inverse_homogenization(
+        ihd, batch, res=r, objective="stiffness:0.5,thermal:0.5", init_mode="rho", volfrac=0.25, penal=3.0,
+        rho_min=1e-9, move=0.2, xmin=1e-3, iters=iters, seed=0, symmetry="none", lambda_print=0.9,
+        overhang_deg=45.0, lambda_cavity=0.9, min_channel_rad=1, cavity_iters=24, cavity_tiles=tiles,
+        filter_k=3, output_dir=str(tmp_path), verbose=False)
+    x_init = np.clip(((rho0 - 1e-9) / (1 - 1e-9)).clip(0, 1) ** (1 / 3), 1e-3, 1)
+    cav = dict(min_channel_rad=1, iters=24)
+    if tiles > 1:
+        cav["tiles_xyz"] = (tiles,) * 3
+    xo, rows = odrv.run_driver(r, x_init, {"stiffness": 0.5, "thermal": 0.5}, iters, 0.25, rtol=1e-11,
+                               lambda_print=0.9, lambda_cavity=0.9, cavity=cav)
+    with open(res.log_csv) as f:
+        log = list(csv.reader(f))[1:]
+    assert len(log) == len(rows) == iters
+    for got, want in zip(log, rows):
+        assert abs(float(got[1]) - want[1]) <= 1e-8 * abs(want[1])        # J_total
+        assert abs(float(got[2]) - want[7]) <= 1e-8 * abs(want[7])        # J_raw
+        assert abs(float(got[3]) - want[8]) <= 1e-9 * max(want[8], 1e-12)  # P_print
+        assert abs(float(got[4]) - want[9]) <= 1e-9 * max(want[9], 1e-12)  # P_cavity
+        assert abs(float(got[7]) - want[4]) <= 1e-7                        # dx_max
+    assert float(log[0][3]) > 0
