@@ -30,13 +30,18 @@ from .cells import CellBatch, hex8_matrices, max_levels
 
 
 class _Homogenize(torch.autograd.Function):
+    """C_H (kappa_H) of the first cell of a batch of `nb` cells, as the reference's
+    TO_homo_3d / TO_homo_1d return it (homogenized_batch_3d(...)[0], model.py:128, 158);
+    every cell of the batch is solved (the reference solves the whole batch too)."""
+
     @staticmethod
-    def forward(ctx, rho, F, owner, physics):
+    def forward(ctx, rho, F, owner, physics, nb):
         ctx.owner = owner
         ctx.physics = physics
+        ctx.nb = nb
         rho64 = rho.detach().reshape(-1).to(torch.float64).contiguous()
-        out = owner._solve(rho64, physics)
-        ctx.gen = owner.cells.generation[physics]
+        out = owner._solve(rho64, physics, nb)
+        ctx.gen = owner.batch(nb).generation[physics]
         ctx.save_for_backward(rho64)
         ctx.rho_dtype = rho.dtype
         ctx.rho_shape = rho.shape
@@ -45,15 +50,19 @@ class _Homogenize(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad):
         (rho64,) = ctx.saved_tensors
-        owner, physics = ctx.owner, ctx.physics
-        if owner.cells.generation[physics] != ctx.gen:
-            owner._solve(rho64, physics)          # the context moved on: re-solve this rho
+        owner, physics, nb = ctx.owner, ctx.physics, ctx.nb
+        cells = owner.batch(nb)
+        if cells.generation[physics] != ctx.gen:
+            owner._solve(rho64, physics, nb)      # the context moved on: re-solve this rho
         g = grad.detach().to(torch.float64).contiguous()
+        m = 6 if physics == "elastic" else 3
+        G = torch.zeros(nb, m, m, dtype=torch.float64, device=g.device)
+        G[0] = g.reshape(m, m)                    # only the first cell's C_H is returned
         if physics == "elastic":
-            d = owner.cells.sensitivity(G6=g.reshape(1, 6, 6))
+            d = cells.sensitivity(G6=G)
         else:
-            d = owner.cells.sensitivity(G3=g.reshape(1, 3, 3))
-        return d.reshape(ctx.rho_shape).to(ctx.rho_dtype), None, None, None
+            d = cells.sensitivity(G3=G)
+        return d.reshape(ctx.rho_shape).to(ctx.rho_dtype), None, None, None, None
 
 
 class IHD(nn.Module):
@@ -84,34 +93,58 @@ class IHD(nn.Module):
         self.maxit = int(maxit)
         self.cells = CellBatch([self.r], nlevels=self.nlevels, device=self.device, precision=precision,
                                omega=omega, nsmooth=nsmooth, warm_start=warm_start, islands=islands)
+        self._precision, self._omega, self._nsmooth = precision, omega, nsmooth
+        self._warm, self._islands = warm_start, islands
+        self._batches = {}
+        self._custom = None
         if (E0, nu, kappa0) != (1.0, 0.3, 1.0):
-            self.cells.set_element_matrices(0, Ke24, Fe24, X024, Ke8, Fe8, X08)
+            self._custom = (Ke24, Fe24, X024, Ke8, Fe8, X08)
+            self.cells.set_element_matrices(0, *self._custom)
         self.last_iters = None
 
-    def _solve(self, rho64: torch.Tensor, physics: str) -> torch.Tensor:
-        CH, kap, it = self.cells.homogenize(rho64, (physics,), rtol=self.rtol, maxit=self.maxit)
+    def batch(self, nb: int) -> CellBatch:
+        """The context of `nb` cells of this resolution (made on first use, then kept)."""
+        nb = int(nb)
+        if nb == 1:
+            return self.cells
+        cb = self._batches.get(nb)
+        if cb is None:
+            cb = CellBatch([self.r] * nb, nlevels=self.nlevels, device=self.device, precision=self._precision,
+                           omega=self._omega, nsmooth=self._nsmooth, warm_start=self._warm, islands=self._islands)
+            for i in range(nb):
+                if self._custom is not None:
+                    cb.set_element_matrices(i, *self._custom)
+            self._batches[nb] = cb
+        return cb
+
+    def _solve(self, rho64: torch.Tensor, physics: str, nb: int = 1) -> torch.Tensor:
+        CH, kap, it = self.batch(nb).homogenize(rho64, (physics,), rtol=self.rtol, maxit=self.maxit)
         self.last_iters = it
         return CH[0] if physics == "elastic" else kap[0]
 
     def _check_batch(self, batch, F, ncase, batch_size):
-        if int(batch_size) != 1:
-            raise ValueError("this drop-in holds one cell per IHD; use maple_amd.CellBatch for batches")
+        nb = int(batch_size)
+        if nb < 1:
+            raise ValueError("batch_size must be >= 1")
         rho = batch["rho"]
-        if rho.numel() != self.r ** 3:
-            raise ValueError(f"batch['rho'] has {rho.numel()} values, expected {self.r ** 3}")
-        if F is not None and F.numel() not in (self.r ** 3 * ncase,):
-            raise ValueError(f"F has {F.numel()} values, expected {self.r ** 3 * ncase}")
+        if rho.numel() != nb * self.r ** 3:
+            raise ValueError(f"batch['rho'] has {rho.numel()} values, expected {nb * self.r ** 3} "
+                             f"({nb} cells of {self.r}^3)")
+        if F is not None and F.numel() != nb * self.r ** 3 * ncase:
+            raise ValueError(f"F has {F.numel()} values, expected {nb * self.r ** 3 * ncase}")
         if rho.device != self.device:
             raise ValueError(f"batch['rho'] is on {rho.device}, IHD is on {self.device}")
-        return rho
+        return rho, nb
 
     def TO_homo_3d(self, batch, Ke, F, X0, batch_size):
-        rho = self._check_batch(batch, F, 18, batch_size)
-        return _Homogenize.apply(rho, F, self, "elastic")
+        """C_H (6, 6) of the first cell of the batch (model.py:98-130)."""
+        rho, nb = self._check_batch(batch, F, 18, batch_size)
+        return _Homogenize.apply(rho, F, self, "elastic", nb)
 
     def TO_homo_1d(self, batch, Ke, F, X0, batch_size):
-        rho = self._check_batch(batch, F, 3, batch_size)
-        return _Homogenize.apply(rho, F, self, "thermal")
+        """kappa_H (3, 3) of the first cell of the batch (model.py:131-159)."""
+        rho, nb = self._check_batch(batch, F, 3, batch_size)
+        return _Homogenize.apply(rho, F, self, "thermal", nb)
 
 
 def make_cfg(resolution: int, smooth_iter=(2, 2, 3, 4), max_cycle: int = 1):
