@@ -93,7 +93,7 @@ def _x0(ncells):
 
 def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", warm: bool = False,
              learned: bool = False, block: bool = False, islands: bool = False, groups: int = 1, refresh: int = 1,
-             maxit: int = 5000, profile_kernel: bool = True):
+             maxit: int = 5000, profile_kernel: bool = True, tune: bool = True):
     """groups > 1: the cells advance in that many concurrent groups (maple_amd.concurrent),
     each its own batch, stream and host thread; every cell's trajectory is unchanged."""
     import numpy as np
@@ -104,6 +104,7 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
 
     dev = torch.device("cuda", 0)
     per = ncells // groups
+    tuned = {}
 
     def make():
         out = []
@@ -111,6 +112,9 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
             cells = maple_amd.CellBatch([R] * per, nlevels=NLEVELS, device=dev, warm_start=warm,
                                         precision=precision, block_cg=block, islands=islands)
             cells.set_coarse_refresh(refresh)
+            if tune:
+                from maple_amd import tune as mt
+                tuned.update({k: v for k, (v, _) in mt.autotune(cells).items()})
             if groups > 1:
                 # the groups already overlap on the device: each solves its two physics in turn
                 # on its own stream (4 hardware queues per process)
@@ -182,7 +186,7 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
     out = {"value": ncells * steps / dt, "ms_per_step": dt * 1e3 / steps, "cells": ncells, "groups": groups,
            "iters_el_max": int(it_el.max()), "iters_el_mean": float(it_el.mean()),
            "iters_th_mean": float(it_th.mean()), "ch_rel_err": err, "J_last": J_last,
-           "last_starts": src}
+           "last_starts": src, "tuned_variants": tuned}
     if count:
         # launches skip converged cells and load cases: the work of each launch is
         # counted on the device (elements x load cases it processed)
@@ -211,6 +215,8 @@ def run_config4(steps: int, precision: str = "mixed", maxit: int = 3000):
     cells = maple_amd.CellBatch([c["r"] for c in spec], nlevels=NLEVELS, device=dev, precision=precision,
                                 warm_start=True, islands=True)
     cells.set_coarse_refresh(3)
+    from maple_amd import tune as mt
+    tuned = {k: v for k, (v, _) in mt.autotune(cells).items()}
     drv = maple_amd.OCDriver(cells, [c["weights"] for c in spec], [c["volfrac"] for c in spec], move=0.2, xmin=1e-3,
                              filter_k=3, rtol=1e-6, maxit=maxit, physics=("elastic", "thermal"), strict=False)
     x = torch.from_numpy(np.concatenate([c["x0"] for c in spec])).to(dev)
@@ -227,7 +233,7 @@ def run_config4(steps: int, precision: str = "mixed", maxit: int = 3000):
     cells.close()
     return {"steps": steps, "ms_per_step": dt * 1e3 / steps, "cells": len(spec), "elements": elems,
             "cell_oc_iters_per_s": len(spec) * steps / dt, "equiv_64cube_oc_iters_per_s": elems / R ** 3 * steps / dt,
-            "unconverged_solves": unconv, "max_cg_iters": worst,
+            "unconverged_solves": unconv, "max_cg_iters": worst, "tuned_variants": tuned,
             "form": "one batch, mixed precision, warm start, island correction, coarsest inverse every 3rd density"}
 
 
@@ -436,7 +442,7 @@ def main():
                    "levels": NLEVELS, "rtol": 1e-6},
         "ch_rel_err_vs_fp64_tight_solve": g["ch_rel_err"],
         "forms": {f: {k: res[f][k] for k in ("value", "ms_per_step", "cells", "groups", "ch_rel_err", "iters_el_mean",
-                                             "iters_th_mean", "iters_el_max", "last_starts")}
+                                             "iters_th_mean", "iters_el_max", "last_starts", "tuned_variants")}
                   | {"kernel_avg_us": res[f].get("kernel", {}).get("avg_us")} for f in forms},
     }
     if "plain" in res and head != "plain":

@@ -162,10 +162,11 @@ int maple_set_coarse_refresh(maple_ctx* ctx, int every);
  * or more blocks, 0 below; the default). */
 /* MAPLE_KERNEL_THERMAL_OPERATOR: the level-0 thermal operator
  * (EBEOperatorScalar.matvec, EBE_GMG_scalar.py:292-306), variant 0 or 1 (element
- * rows per pencil lane, as variants 0 and 1 above) or 2 (node form: all load cases
- * per thread, z neighbours from the neighbouring lanes, the element matrix's four
- * Hamming-distance entries instead of 64; cells with nz = 64 whose thermal element
- * matrix is a Q1 Laplacian, else variant 0 runs).  Default: 2. */
+ * rows per pencil lane, as variants 0 and 1 above), 2 or 3 (node form: all load
+ * cases per thread, z neighbours from the neighbouring lanes, the element matrix's
+ * four Hamming-distance entries instead of 64; 2: two x-adjacent nodes per thread,
+ * 3: one; cells with nz = 64 whose thermal element matrix is a Q1 Laplacian, else
+ * variant 0 runs).  Default: 2. */
 #define MAPLE_KERNEL_FINE_OPERATOR 0
 #define MAPLE_KERNEL_COARSE_OPERATOR 1
 #define MAPLE_KERNEL_THERMAL_OPERATOR 2

@@ -399,8 +399,10 @@ Launch op_launch(maple_ctx* c, const Phys& P, bool full) {
   if (P.nd == 1) {
     // thermal: node form (nz = 64 cells, Q1-Laplacian element matrices) or element rows
     int tv = c->thermal_var;
-    if (tv == 2 && !(c->all_nz64 && P.laplacian)) tv = 0;
-    Launch L = tv == 2 ? (full ? c->lv[0].pair : P.lx) : (full ? c->lv[0].fine : P.lf);
+    if (tv >= 2 && !(c->all_nz64 && P.laplacian)) tv = 0;
+    Launch L = tv == 2 ? (full ? c->lv[0].pair : P.lx)
+             : tv == 3 ? (full ? c->lv[0].node : P.ln[0])
+                       : (full ? c->lv[0].fine : P.lf);
     L.var = tv;
     return L;
   }
@@ -432,7 +434,7 @@ void apply_fine64(maple_ctx* c, Phys& P, int mode, const double* rho, const doub
     fine_apply_node<double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
   else if (P.nd == 3)
     fine_apply<3, double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
-  else if (L.var == 2)
+  else if (L.var >= 2)
     thermal_apply64<double>(L, mode, P.nc, P.lap, rho, u, f, dinv, c->omega, out, partial, s);
   else
     fine_apply<1, double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
@@ -455,7 +457,7 @@ void apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const T* u, const
       fine_apply_node<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
     else if (P.nd == 3)
       fine_apply<3, T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
-    else if (L.var == 2)
+    else if (L.var >= 2)
       thermal_apply64<T>(L, mode, P.nc, P.lap, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
     else
       fine_apply<1, T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
@@ -1111,7 +1113,7 @@ int maple_set_variant(maple_ctx* c, int kernel, int variant) {
     return 0;
   }
   if (kernel == MAPLE_KERNEL_THERMAL_OPERATOR) {
-    if (variant < 0 || variant > 2) return fail(MAPLE_EINVAL, "unknown variant");
+    if (variant < 0 || variant > 3) return fail(MAPLE_EINVAL, "unknown variant");
     c->thermal_var = variant;
     return 0;
   }

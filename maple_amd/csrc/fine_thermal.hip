@@ -45,15 +45,16 @@ __device__ __forceinline__ T s_sum(const T (&r)[2][2][2], int dx, int dy, int dz
 }  // namespace
 
 // MODE: 0 = K u, 1 = K u and partial (u, K u), 2 = f - K u, 3 = u + omega Dinv (f - K u).
-// Thread t of a cell owns nodes (2 gx, y, z) and (2 gx + 1, y, z), t = (gx*ny + y)*64 + z
-// (the pair launch of the level-0 grid).
-template <typename T, int MODE>
+// Thread t of a cell owns the NX nodes (NX gx + k, y, z), t = (gx*ny + y)*64 + z: NX = 2 on
+// the pair launch of the level-0 grid (neighbour columns shared by two nodes), NX = 1 on
+// the node launch (half the live registers, twice the waves in flight).
+template <typename T, int MODE, int NX>
 __global__ __launch_bounds__(256) void k_thermal64(
     const LevelCell* __restrict__ cells, const int* __restrict__ blk_start, int ncells, long long Ntot,
     int ncase, const double* __restrict__ coef, const T* __restrict__ rho, const T* __restrict__ u,
     const T* __restrict__ f, const T* __restrict__ dinv, T omega, T* __restrict__ out, double* __restrict__ partial,
     const int* __restrict__ done, int ndc, unsigned long long* __restrict__ work) {
-  constexpr int NX = 2, NC = 3;
+  constexpr int NC = 3;
   count_work(cells, ncells, done, ndc, ncase, work);
   const int k = find_cell(blk_start, ncells, lbid());
   const LevelCell c = cells[k];
@@ -164,15 +165,23 @@ __global__ __launch_bounds__(256) void k_thermal64(
   if constexpr (MODE == 1) block_sum<NC>(dj, partial + (long long)lbid() * MAPLE_MAX_CASES);
 }
 
+template <typename T, int NX>
+void thermal_apply64_g(const Launch& L, int mode, int ncase, const double* coef, const T* rho, const T* u,
+                       const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s) {
+  switch (mode) {
+    case 0: k_thermal64<T, 0, NX><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    case 1: k_thermal64<T, 1, NX><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    case 2: k_thermal64<T, 2, NX><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    default: k_thermal64<T, 3, NX><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+  }
+}
+
+// L.var 2: two nodes per thread (pair launch); 3: one node per thread (node launch)
 template <typename T>
 void thermal_apply64(const Launch& L, int mode, int ncase, const double* coef, const T* rho, const T* u, const T* f,
                      const T* dinv, T omega, T* out, double* partial, hipStream_t s) {
-  switch (mode) {
-    case 0: k_thermal64<T, 0><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
-    case 1: k_thermal64<T, 1><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
-    case 2: k_thermal64<T, 2><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
-    default: k_thermal64<T, 3><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
-  }
+  if (L.var == 3) thermal_apply64_g<T, 1>(L, mode, ncase, coef, rho, u, f, dinv, omega, out, partial, s);
+  else thermal_apply64_g<T, 2>(L, mode, ncase, coef, rho, u, f, dinv, omega, out, partial, s);
 }
 template void thermal_apply64<double>(const Launch&, int, int, const double*, const double*, const double*, const double*, const double*, double, double*, double*, hipStream_t);
 template void thermal_apply64<float>(const Launch&, int, int, const double*, const float*, const float*, const float*, const float*, float, float*, double*, hipStream_t);

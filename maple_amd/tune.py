@@ -1,20 +1,35 @@
 """Kernel-variant tuner for a CellBatch.
 
-For every tunable kernel the tuner times each variant on the batch's own
-shapes (the real cell sizes, all load cases, HIP events on the batch's
-stream) and keeps the fastest.  The choice depends on the cell mix: the
-register-staged level-0 operator wins when occupancy is the limit, the
-all-in-registers one when L1 re-loads are; the node-block forms (six or three
-load cases per thread) when the arithmetic is.
+For every tunable kernel the tuner times each variant on the batch's own shapes
+(its real cell sizes, every load case, on the batch's stream) in the roles the
+solver gives it, and keeps the fastest:
+
+* ``fine_operator``: the level-0 elastic operator, one CG product (FP64) plus one
+  smoother sweep of the preconditioner's precision (FP32 in the mixed form);
+* ``thermal_operator``: the level-0 thermal smoother sweep;
+* ``coarse_operator``: the smoother sweeps of every coarse stencil level.
+
+The choice depends on the cell mix (for a batch with cells whose nz is not 64 the
+lane-rotation variants do not apply and the 2x1 node blocks usually win), so it is
+kept per batch shape (cell sizes, precision, level count) for the process:
+``autotune(cells)`` on a second batch of the same shape reuses it.
 """
 from __future__ import annotations
+
+from typing import Dict, Tuple
 
 import torch
 
 from . import _lib as L
 from .cells import CellBatch
 
-KERNELS = {"fine_operator": (L.MAPLE_KERNEL_FINE_OPERATOR, (0, 1, 2, 3, 4, 5, 6, 7, 8))}
+KERNELS = {
+    "fine_operator": (L.MAPLE_KERNEL_FINE_OPERATOR, (0, 1, 2, 3, 4, 5, 6, 7, 8)),
+    "thermal_operator": (L.MAPLE_KERNEL_THERMAL_OPERATOR, (0, 1, 2, 3)),
+    "coarse_operator": (L.MAPLE_KERNEL_COARSE_OPERATOR, (0, 1, 2)),
+}
+
+_CACHE: Dict[Tuple, Dict[str, int]] = {}
 
 
 def _time(fn, reps: int) -> float:
@@ -30,18 +45,50 @@ def _time(fn, reps: int) -> float:
     return e0.elapsed_time(e1) / reps
 
 
-def autotune(cells: CellBatch, reps: int = 10, seed: int = 0) -> dict:
-    """Pick the fastest variant of every tunable kernel for this batch; returns {kernel: (variant, ms)}."""
+def shape_key(cells: CellBatch) -> Tuple:
+    return (tuple(cells.dims), cells.precision, cells.nlevels)
+
+
+def autotune(cells: CellBatch, reps: int = 5, seed: int = 0, force: bool = False) -> dict:
+    """Pick the fastest variant of every tunable kernel for this batch shape and set it;
+    returns {kernel: (variant, {variant: ms})} ({kernel: (variant, None)} from the cache)."""
+    key = shape_key(cells)
+    if not force and key in _CACHE:
+        for name, v in _CACHE[key].items():
+            cells.set_variant(KERNELS[name][0], v)
+        return {name: (v, None) for name, v in _CACHE[key].items()}
     g = torch.Generator(device=cells.device).manual_seed(seed)
-    rho = torch.rand(cells.nelem, generator=g, device=cells.device, dtype=torch.float64) + 1e-3
-    u = torch.randn(6, 3, cells.nelem, generator=g, device=cells.device, dtype=torch.float64)
-    out = {}
+    dev, n = cells.device, cells.nelem
+    x = torch.rand(n, generator=g, device=dev, dtype=torch.float64)
+    rho = 1e-9 + (1 - 1e-9) * (0.05 + 0.95 * x) ** 3
+    pre = cells.precision
+    keep = []
+    for phys in ("elastic", "thermal"):
+        keep.append(cells.mg_setup(phys, rho, "fp64"))
+        if pre != "fp64":
+            cells.mg_setup(phys, rho, pre)
+    u = torch.randn(6, 3, n, generator=g, device=dev, dtype=torch.float64)
+    ut = torch.randn(3, 1, n, generator=g, device=dev, dtype=torch.float64)
+    uc = {l: torch.randn(6, 3, cells.num_nodes(l), generator=g, device=dev, dtype=torch.float64)
+          for l in range(1, cells.nlevels - 1)}
+    roles = {
+        "fine_operator": lambda: (cells.apply_operator("elastic", rho, u),
+                                  cells.mg_apply("elastic", 0, u, u, mode=3, precision=pre)),
+        "thermal_operator": lambda: cells.mg_apply("thermal", 0, ut, ut, mode=3, precision=pre),
+        "coarse_operator": lambda: [cells.mg_apply("elastic", l, uc[l], uc[l], mode=3, precision=pre) for l in uc],
+    }
+    out, chosen = {}, {}
     for name, (kid, variants) in KERNELS.items():
+        if name == "coarse_operator" and not uc:
+            continue
         times = {}
         for v in variants:
             cells.set_variant(kid, v)
-            times[v] = _time(lambda: cells.apply_operator("elastic", rho, u), reps)
+            times[v] = _time(roles[name], reps)
         best = min(times, key=times.get)
         cells.set_variant(kid, best)
         out[name] = (best, times)
+        chosen[name] = best
+    _CACHE[key] = chosen
+    del keep
     return out

@@ -340,6 +340,39 @@ def test_ihd_dropin_autograd():
     assert np.abs(f.cpu().numpy().ravel() - ooc.box_filter(go, (r, r, r), 3)).max() <= 1e-14 * np.abs(go).max()
 
 
+def test_ihd_dropin_batch_of_cells():
+    """batch_size > 1: every cell is solved and, as the reference's TO_homo_3d /
+    TO_homo_1d return homogenized_batch_3d(...)[0] (model.py:128, 158), the result is
+    the first cell's C_H / kappa_H; its gradient is the single-cell one on the first
+    cell's elements and zero on the others."""
+    import maple_amd
+    r, nb = 8, 3
+    ihd = maple_amd.IHD(maple_amd.make_cfg(r, smooth_iter=[2, 2, 3]), DEV, rtol=1e-11)
+    rng = np.random.default_rng(4)
+    x = torch.from_numpy(np.clip(0.3 + rng.uniform(-0.25, 0.25, nb * r ** 3), 1e-3, 1)).to(DEV)
+    outs = {}
+    for m in (nb, 1):
+        xv = x[:m * r ** 3].reshape(-1, 1).clone().requires_grad_(True)
+        rho = 1e-9 + (1 - 1e-9) * xv ** 3
+        batch = {"rho": rho}
+        Fm = torch.zeros(m * r ** 3 * 18, dtype=torch.float64, device=DEV)
+        Ft = torch.zeros(m * r ** 3 * 3, dtype=torch.float64, device=DEV)
+        CH = ihd.TO_homo_3d(batch, ihd.Ke_3d, Fm, ihd.X0_3d, batch_size=m)
+        kap = ihd.TO_homo_1d(batch, ihd.Ke_1d, Ft, ihd.X0_1d, batch_size=m)
+        assert CH.shape == (6, 6) and kap.shape == (3, 3)
+        J = maple_amd.objective_from_spec(obj="stiffness:0.5,thermal:0.5", CH=CH, kappa=kap)
+        (g,) = torch.autograd.grad(J, xv)
+        outs[m] = (CH.detach().cpu().numpy(), kap.detach().cpu().numpy(), g.cpu().numpy().ravel())
+    CHb, kb, gb = outs[nb]
+    CH1, k1, g1 = outs[1]
+    assert np.linalg.norm(CHb - CH1) <= 1e-10 * np.linalg.norm(CH1)
+    assert np.linalg.norm(kb - k1) <= 1e-10 * np.linalg.norm(k1)
+    assert np.abs(gb[:r ** 3] - g1).max() <= 1e-9 * np.abs(g1).max()
+    assert np.all(gb[r ** 3:] == 0.0)
+    with pytest.raises(ValueError):
+        ihd.TO_homo_3d({"rho": x.reshape(-1, 1)}, ihd.Ke_3d, None, ihd.X0_3d, batch_size=2)
+
+
 def test_batched_oc_steps_equal_single_cells():
     """A mixed-size batch with per-cell objectives and volume fractions advances exactly like
     the same cells run one by one (convergence-aware block skipping included)."""
@@ -380,7 +413,7 @@ def test_concurrent_physics_equal_sequential():
     assert torch.equal(da, db)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("r,ncase", [(64, 3), (64, 2), (32, 3)])
 def test_thermal_operator_variants(r, ncase, variant):
     """The thermal level-0 operator: element rows (variants 0, 1) and the node form
