@@ -154,7 +154,11 @@ int maple_set_coarse_refresh(maple_ctx* ctx, int every);
  * which gather their shared neighbours once), 6 or 7 (2x2 nodes in x and z per
  * thread, three or two load cases) or 8 (as 5 with the z neighbours taken from
  * the neighbouring lanes of a wavefront, which holds one z-pencil; cells with
- * nz = 64, else 5 is used).  Default: 8. */
+ * nz = 64, else 5 is used) or 9 (the FP32 operator of the mixed form's
+ * preconditioner as element products Ke (rho_e u_e) on the matrix cores,
+ * v_mfma_f32_16x16x4f32, one wavefront per node pencil marching along x; cubic
+ * cells with r a multiple of 16 up to 64; the FP64 operator runs as 8, or 5).
+ * Default: 8. */
 /* MAPLE_KERNEL_COARSE_OPERATOR: the elastic Galerkin stencil product of the
  * coarse levels (the coarse matvec of the V-cycle, EBE_GMG.py:886-984), variant
  * 0 (one thread per node row, three times the waves), 1 (one thread per node,

@@ -316,7 +316,9 @@ class CellBatch:
 
     # -- multigrid internals -------------------------------------------------
     def _prec(self, precision: str) -> int:
-        return {"fp64": L.MAPLE_PREC_FP64, "mixed": L.MAPLE_PREC_MIXED}[precision]
+        """Hierarchy of the multigrid internals: "fp64", or the mixed form's FP32 one
+        ("mixed", also spelled "fp32")."""
+        return {"fp64": L.MAPLE_PREC_FP64, "mixed": L.MAPLE_PREC_MIXED, "fp32": L.MAPLE_PREC_MIXED}[precision]
 
     def mg_setup(self, physics: str, rho: torch.Tensor, precision: str = "fp64"):
         """Build the hierarchy of rho without solving (the FP64 level 0 keeps reading

@@ -51,8 +51,9 @@ struct Level {
   int* doblk = nullptr;   // level 0: 1024-element blocks (OC)
   int* dxblk = nullptr;   // level 0: 256 node pairs per block (2x1 node-block operator)
   int* dqblk = nullptr;   // level 0: 256 node quads per block (2x2 node-block operator)
-  Launch node, fine, oc, pair, quad;
-  std::vector<int> cnt_node, cnt_fine, cnt_pair, cnt_quad;   // blocks per cell of the launches
+  int* dmblk = nullptr;   // level 0: 4 node pencils x 16 planes per block (MFMA operator)
+  Launch node, fine, oc, pair, quad, mfma;
+  std::vector<int> cnt_node, cnt_fine, cnt_pair, cnt_quad, cnt_mfma;   // blocks per cell of the launches
 };
 
 // Multigrid hierarchy and V-cycle work vectors in precision T (FP64 for the
@@ -85,7 +86,7 @@ struct Phys {
   int* done_h = nullptr;   // pinned, 2 slots of ncells*nc + 1
   std::vector<Launch> ln;  // per-level node launches that skip converged (cell, case) pairs
   Launch lf;               // level-0 operator launch, same
-  Launch lx, lq;           // level-0 node-pair and node-quad operator launches, same
+  Launch lx, lq, lm;       // level-0 node-pair, node-quad and MFMA operator launches, same
   double* atol2 = nullptr;
   bool have_fields = false, have_energy = false;
   double* ustart = nullptr;   // supplied start of the next solve
@@ -102,6 +103,8 @@ struct Phys {
   int* cbx = nullptr;
   LevelCell* ccq = nullptr;
   int* cbq = nullptr;
+  LevelCell* ccm = nullptr;
+  int* cbm = nullptr;
   char* stage = nullptr;
   size_t stage_bytes = 0;
   int stage_slots = 0, stage_next = 0, nlive = 0;
@@ -122,6 +125,7 @@ struct Timer {
 struct maple_ctx {
   int device = 0, ncells = 0, nlevels = 0, prec = 0;
   bool all_nz64 = false;   // every cell has nz = 64 (level-0 operator variant 8 applies)
+  bool mfma_ok = false;    // every cell fits the MFMA operator (variant 9)
   int thermal_var = 2;     // thermal level-0 operator: 0/1 element rows, 2 node form where it applies
   std::vector<int> dims;
   std::vector<Level> lv;
@@ -182,6 +186,7 @@ int set_device(const maple_ctx* c) {
   return 0;
 }
 
+// per_block == 0: blocks per cell from the MFMA operator's wavefront split
 int make_launch(maple_ctx* c, Level& L, int per_block, bool fine_waves_mode, int** dptr, Launch* out,
                 std::vector<int>* counts = nullptr) {
   std::vector<int> blk(c->ncells + 1, 0);
@@ -189,7 +194,8 @@ int make_launch(maple_ctx* c, Level& L, int per_block, bool fine_waves_mode, int
   for (int i = 0; i < c->ncells; ++i) {
     const LevelCell& lc = L.h[i];
     long long n = fine_waves_mode ? (long long)fine_waves(lc) : (long long)lc.nx * lc.ny * lc.nz;
-    blk[i + 1] = blk[i] + (int)((n + per_block - 1) / per_block);
+    if (per_block == 0) blk[i + 1] = blk[i] + (fine_mfma_applies(lc) ? fine_mfma_blocks(lc) : 0);
+    else blk[i + 1] = blk[i] + (int)((n + per_block - 1) / per_block);
     if (counts) (*counts)[i] = blk[i + 1] - blk[i];
   }
   TRY(dalloc(c, dptr, blk.size()));
@@ -341,6 +347,9 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
   P.lq = c->lv[0].quad;
   P.lq.done = P.done_d;
   P.lq.ndc = nc;
+  P.lm = c->lv[0].mfma;
+  P.lm.done = P.done_d;
+  P.lm.ndc = nc;
   P.cc.assign(L, nullptr);
   P.cb.assign(L, nullptr);
   for (int l = 0; l < L; ++l) {
@@ -353,8 +362,10 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
   TRY(dalloc(c, &P.cbx, (size_t)c->ncells + 1));
   TRY(dalloc(c, &P.ccq, (size_t)c->ncells));
   TRY(dalloc(c, &P.cbq, (size_t)c->ncells + 1));
+  TRY(dalloc(c, &P.ccm, (size_t)c->ncells));
+  TRY(dalloc(c, &P.cbm, (size_t)c->ncells + 1));
   auto r8 = [](size_t b) { return (b + 7) / 8 * 8; };
-  P.stage_bytes = (size_t)(L + 3) * (r8(sizeof(LevelCell) * c->ncells) + r8(sizeof(int) * (c->ncells + 1)));
+  P.stage_bytes = (size_t)(L + 4) * (r8(sizeof(LevelCell) * c->ncells) + r8(sizeof(int) * (c->ncells + 1)));
   P.stage_slots = c->ncells + 2;   // a solve compacts at most ncells - 1 times: no slot is reused in flight
   void* sp = nullptr;
   HIPCHK(hipHostMalloc(&sp, P.stage_bytes * P.stage_slots, hipHostMallocDefault));
@@ -394,8 +405,9 @@ void timed_end(maple_ctx* c, int kind, hipStream_t s) {
 
 // Launch of the level-0 operator of physics P: the pencil launch of the element-row
 // kernels, or the node launch of the node-block kernels (elastic, variants 2-3).
-// full: every cell and case (no convergence flags).
-Launch op_launch(maple_ctx* c, const Phys& P, bool full) {
+// full: every cell and case (no convergence flags).  fp32: the FP32 operator (variant 9
+// applies); the FP64 operator runs variant 9's choice as variant 8 (or 5).
+Launch op_launch(maple_ctx* c, const Phys& P, bool full, bool fp32 = false) {
   if (P.nd == 1) {
     // thermal: node form (nz = 64 cells, Q1-Laplacian element matrices) or element rows
     int tv = c->thermal_var;
@@ -407,9 +419,11 @@ Launch op_launch(maple_ctx* c, const Phys& P, bool full) {
     return L;
   }
   int v = c->lv[0].fine.var;
+  if (v == 9 && !(fp32 && c->mfma_ok)) v = 8;            // FP32 on the matrix cores only
   if (v == 8 && !c->all_nz64) v = 5;                     // the lane-rotation form needs nz = 64
   const int var = v;
-  Launch L = var == 8 ? (full ? c->lv[0].pair : P.lx)
+  Launch L = var == 9 ? (full ? c->lv[0].mfma : P.lm)
+           : var == 8 ? (full ? c->lv[0].pair : P.lx)
            : var >= 6 ? (full ? c->lv[0].quad : P.lq)
            : var >= 4 ? (full ? c->lv[0].pair : P.lx)
            : var >= 2 ? (full ? c->lv[0].node : P.ln[0])
@@ -446,9 +460,16 @@ template <typename T>
 void apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const T* u, const T* f, T* out, hipStream_t s) {
   if (l == 0) {
     const int kind = P.nd == 3 ? (sizeof(T) == 8 ? 0 : 2) : (sizeof(T) == 8 ? 1 : -1);
-    Launch L = op_launch(c, P, false);
+    Launch L = op_launch(c, P, false, sizeof(T) == 4);
     L.work = c->timing && kind >= 0 ? c->work_d + kind : nullptr;
     if (kind >= 0) timed_begin(c, kind, s);
+    if constexpr (sizeof(T) == 4) {
+      if (P.nd == 3 && L.var == 9) {
+        fine_apply_mfma(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, s);
+        if (kind >= 0) timed_end(c, kind, s);
+        return;
+      }
+    }
     if (P.nd == 3 && L.var == 8)
       fine_apply_nodeb64<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
     else if (P.nd == 3 && L.var >= 4)
@@ -617,6 +638,9 @@ void full_launches(maple_ctx* c, Phys& P) {
   P.lq = c->lv[0].quad;
   P.lq.done = P.done_d;
   P.lq.ndc = P.nc;
+  P.lm = c->lv[0].mfma;
+  P.lm.done = P.done_d;
+  P.lm.ndc = P.nc;
   P.nlive = c->ncells;
 }
 
@@ -658,6 +682,7 @@ int compact_launches(maple_ctx* c, Phys& P, const int* done_h, hipStream_t s) {
   TRY(build(c->lv[0], c->lv[0].cnt_fine, P.ccf, P.cbf, P.lf));
   TRY(build(c->lv[0], c->lv[0].cnt_pair, P.ccx, P.cbx, P.lx));
   TRY(build(c->lv[0], c->lv[0].cnt_quad, P.ccq, P.cbq, P.lq));
+  if (c->mfma_ok) TRY(build(c->lv[0], c->lv[0].cnt_mfma, P.ccm, P.cbm, P.lm));
   P.nlive = n;
   return 0;
 }
@@ -1004,6 +1029,13 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
       if ((rc = make_launch(c, L, 1024, false, &L.doblk, &L.oc))) return bail(rc);
       if ((rc = make_launch(c, L, 512, false, &L.dxblk, &L.pair, &L.cnt_pair))) return bail(rc);
       if ((rc = make_launch(c, L, 1024, false, &L.dqblk, &L.quad, &L.cnt_quad))) return bail(rc);
+      // MFMA operator: every cell must fit it, and the kernel's 32-bit offsets must cover
+      // the 18 FP32 planes of the level (6 cases x 3 components)
+      c->mfma_ok = 18LL * L.Ntot < (1LL << 31);
+      for (int i = 0; i < ncells; ++i) c->mfma_ok = c->mfma_ok && fine_mfma_applies(L.h[i]);
+      if ((rc = make_launch(c, L, 0, false, &L.dmblk, &L.mfma, &L.cnt_mfma))) return bail(rc);
+      for (int i = 0; i < ncells; ++i)
+        if (fine_mfma_applies(L.h[i])) L.mfma.var_mask |= 1 << (L.h[i].nz / 16 - 1);
       // default level-0 elastic operator: 2x1 node blocks, three load cases per thread,
       // z neighbours from the neighbouring lanes when every cell has nz = 64
       L.fine.var = 8;
@@ -1107,7 +1139,7 @@ int maple_set_solver(maple_ctx* c, double omega, int nsmooth, int warm_start) {
 int maple_set_variant(maple_ctx* c, int kernel, int variant) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
   if (kernel == MAPLE_KERNEL_FINE_OPERATOR) {
-    if (variant < 0 || variant > 8) return fail(MAPLE_EINVAL, "unknown variant");
+    if (variant < 0 || variant > 9) return fail(MAPLE_EINVAL, "unknown variant");
     c->lv[0].fine.var = variant;
     for (Phys& P : c->ph) P.lf.var = variant;
     return 0;

@@ -22,8 +22,11 @@ struct Launch {
   // per pencil lane (0: neighbour values in registers, 1: staged element by element)
   // or node blocks (2: six load cases per thread, 3: three; 4, 5: the same with two
   // x-adjacent nodes per thread; 6, 7: 2x2 nodes in x and z, three / two cases;
-  // 8: as 5 with z neighbours taken from the neighbouring lanes, cells with nz = 64)
+  // 8: as 5 with z neighbours taken from the neighbouring lanes, cells with nz = 64;
+  // 9: FP32 element products on the matrix cores, FP64 as 8 or 5)
   int var = 0;
+  // MFMA operator launch: bit t set when the listed cells include cells of nz = 16 (t + 1)
+  int var_mask = 0;
   // level-0 operator timing: when set, the launch adds the elements x load cases it
   // processes (listed cells, cases not converged) to *work
   unsigned long long* work = nullptr;
@@ -65,6 +68,14 @@ void fine_apply_nodeb(const Launch& L, int mode, int ncase, const T* Ke, const T
 template <typename T>
 void fine_apply_nodeb64(const Launch& L, int mode, int ncase, const T* Ke, const T* rho, const T* u,
                         const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s);
+// Variant 9, FP32 only: the element product Ke (rho_e u_e) on v_mfma_f32_16x16x4f32, a
+// wavefront per node pencil marching along x (modes 0, 2, 3).  Cubic cells with
+// r % 16 == 0 and r <= 64 (fine_mfma_applies); L: the launch of fine_mfma_blocks
+// blocks (4 wavefronts) per cell.
+bool fine_mfma_applies(const LevelCell& c);
+int fine_mfma_blocks(const LevelCell& c);
+void fine_apply_mfma(const Launch& L, int mode, int ncase, const float* Ke, const float* rho, const float* u,
+                     const float* f, const float* dinv, float omega, float* out, hipStream_t s);
 // Thermal operator in node form for nz = 64 cells whose element matrix is the Q1
 // Laplacian (entry k[d] for corners at Hamming distance d; coef: 4 per cell), same
 // modes, all (<= 3) load cases per thread; L: the pair launch.

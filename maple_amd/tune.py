@@ -24,7 +24,7 @@ from . import _lib as L
 from .cells import CellBatch
 
 KERNELS = {
-    "fine_operator": (L.MAPLE_KERNEL_FINE_OPERATOR, (0, 1, 2, 3, 4, 5, 6, 7, 8)),
+    "fine_operator": (L.MAPLE_KERNEL_FINE_OPERATOR, (0, 1, 2, 3, 4, 5, 6, 7, 8, 9)),
     "thermal_operator": (L.MAPLE_KERNEL_THERMAL_OPERATOR, (0, 1, 2, 3)),
     "coarse_operator": (L.MAPLE_KERNEL_COARSE_OPERATOR, (0, 1, 2)),
 }
