@@ -76,12 +76,16 @@ PEAK = {"fp64": 78.6, "mixed": 78.6}
 PEAK_PACKED_OR_MFMA = {"fp64": 78.6, "mixed": 157.3}
 FLOP_PER_ELEM_ELASTIC = 1152      # 24 rows x 24 cols x 2 per element and load case
 TRAJ = 100                        # OC iterations of every timed trajectory
-# HBM bytes per element and load case of one level-0 elastic operator launch, from
-# the rocprofv3 FETCH_SIZE and WRITE_SIZE runs of profiles/fine_operator_64cube_hbm_counters.md
-# (copy-calibrated): FP64 product 93.5 MB, FP32 Jacobi sweep 81.5 MB per 64^3 cell and 6 cases
-TRAFFIC_PER_ELEM_CASE = {"fp64": 93.5e6 / (6 * 64 ** 3), "mixed": 81.5e6 / (6 * 64 ** 3)}
-TRAFFIC_SOURCE = "derived from profiles/fine_operator_64cube_hbm_counters.md (FETCH_SIZE + WRITE_SIZE, " \
-                 "copy-calibrated) x the element-cases of the launches timed here"
+# HBM bytes per element and load case of one level-0 elastic operator launch, from the
+# rocprofv3 FETCH_SIZE and WRITE_SIZE runs (copy-calibrated) of the 64^3 operator profiles:
+# FP64 product 93.8 MB per 64^3 cell and 6 cases; FP32 Jacobi sweep of 16 cells x 6 cases
+# (25 165 824 element-cases) 1283.6 MB with the vector kernel, 1087.5 MB on the matrix cores
+TRAFFIC_PER_ELEM_CASE = {("fp64", None): 93.8e6 / (6 * 64 ** 3), ("mixed", 8): 1283.6e6 / 25165824,
+                         ("mixed", 9): 1087.5e6 / 25165824}
+TRAFFIC_SOURCE = {"fp64": "profiles/fine_operator_64cube_hbm_counters.md", "mixed": "profiles/level0_fp32_sweep_16x64cube.md"}
+KERNEL_NAME = {8: "k_fine_nodeb64<{t},*,3> (level-0 elastic operator, vector FMA node blocks, z neighbours by lane rotation",
+               5: "k_fine_nodeb<{t},*,3,2,1> (level-0 elastic operator, vector FMA node blocks",
+               9: "k_fine_mfma<*,4> (level-0 elastic operator, element products on v_mfma_f32_16x16x4f32"}
 
 
 def _x0(ncells):
@@ -193,12 +197,19 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
         avg_s = kms / count * 1e-3
         flops = work / count * FLOP_PER_ELEM_ELASTIC
         tname = "double" if precision == "fp64" else "float"
-        out["kernel"] = {"name": f"k_fine_nodeb64<{tname},*,3> (level-0 elastic operator, node blocks, "
-                                 f"up to 6 load cases x {per} cells per launch)",
+        var = tuned.get("fine_operator", 8)
+        kvar = 8 if (precision == "fp64" and var == 9) else var   # the FP64 operator runs 9 as 8
+        mfma = precision != "fp64" and kvar == 9
+        traffic = TRAFFIC_PER_ELEM_CASE.get(("fp64", None) if precision == "fp64" else ("mixed", kvar))
+        out["kernel"] = {"name": KERNEL_NAME.get(kvar, f"level-0 elastic operator variant {kvar} ({{t}}").format(t=tname)
+                                 + f", up to 6 load cases x {per} cells per launch)",
+                         "variant": kvar, "mfma": mfma,
                          "launches": count, "avg_us": avg_s * 1e6, "flop_per_launch": flops,
                          "elem_cases_per_launch": work / count,
-                         "achieved_tflops": flops / avg_s / 1e12, "peak": PEAK[precision],
-                         "traffic": work / count * TRAFFIC_PER_ELEM_CASE[precision]}
+                         "achieved_tflops": flops / avg_s / 1e12,
+                         "peak": PEAK_PACKED_OR_MFMA[precision] if mfma else PEAK[precision],
+                         "traffic": work / count * traffic if traffic else None,
+                         "traffic_source": TRAFFIC_SOURCE["fp64" if precision == "fp64" else "mixed"]}
     return out
 
 
@@ -455,18 +466,22 @@ def main():
     if k:
         prec = FORMS[rl_form]["precision"]
         line["roofline"] = {
-            "bound": "valu-fp64" if prec == "fp64" else "valu-fp32",
+            "bound": "mfma" if k["mfma"] else ("valu-fp64" if prec == "fp64" else "valu-fp32"),
             "achieved": k["achieved_tflops"], "peak": k["peak"], "unit": "TFLOP/s",
             "frac": k["achieved_tflops"] / k["peak"],
             "frac_of_packed_or_mfma_peak": k["achieved_tflops"] / PEAK_PACKED_OR_MFMA[prec],
-            "traffic": k["traffic"], "traffic_unit": "HBM bytes per launch, " + TRAFFIC_SOURCE,
+            "traffic": k["traffic"],
+            "traffic_unit": "HBM bytes per launch, derived from " + k["traffic_source"] + " (FETCH_SIZE + WRITE_SIZE, "
+                            "copy-calibrated, per element-case) x the element-cases of the launches timed here",
             "kernel": k["name"], "avg_us": k["avg_us"], "launches": k["launches"],
             "elem_cases_per_launch": k["elem_cases_per_launch"],
             "algorithmic_flop_per_launch": k["flop_per_launch"],
             "flop_per_elem_case": FLOP_PER_ELEM_ELASTIC, "measured_on_form": rl_form,
-            "roof": "the kernel issues non-packed vector FMA (no MFMA, packed FP32 disabled in the build): "
-                    "FP64 and non-packed FP32 issue at 78.6 TFLOP/s on MI355X; packed FP32 / FP32 MFMA would "
-                    "reach 157.3"}
+            "roof": ("FP32 MFMA, 157.3 TFLOP/s on MI355X (24 of the 32 rows of the two 16-row tiles per element "
+                     "are Ke rows)") if k["mfma"] else
+                    ("the kernel issues non-packed vector FMA (packed FP32 disabled in the build): FP64 and "
+                     "non-packed FP32 issue at 78.6 TFLOP/s on MI355X; packed FP32 / FP32 MFMA reach 157.3 "
+                     "(the MFMA form, variant 9, is tuned against it)")}
     if not args.no_config4:
         line["config4"] = run_config4(TRAJ)
         print(f"# config4: {line['config4']['equiv_64cube_oc_iters_per_s']:.1f} 64^3-equivalent OC it/s",
