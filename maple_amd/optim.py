@@ -4,10 +4,11 @@
   ``avg_filter3d_x_periodic`` (main_IHD_v3.py:342-359), ``oc_update``
   (main_IHD_v3.py:362-380), ``build_unitcell_batch`` (main_IHD_v3.py:162-215),
   ``assemble_F_fast`` / ``assemble_F_fast_scalar`` (_utils.py:453-507).
-* ``OCDriver``: the plain OC loop of optimize_inverse_homogenization
-  (main_IHD_v3.py:620-716, symmetry "none", no printability penalties) over a
+* ``OCDriver``: the OC loop of optimize_inverse_homogenization
+  (main_IHD_v3.py:620-716, symmetry "none", the AM penalties when given) over a
   ``CellBatch`` of any number of cells, everything on the device except the
-  6x6 objective algebra.
+  6x6 objective algebra (symmetry projection: ``maple_amd.driver`` and
+  ``maple_amd.densify``).
 * Initial designs of the single-cell workloads.
 """
 from __future__ import annotations
