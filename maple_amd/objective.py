@@ -22,65 +22,58 @@ from typing import Dict, Mapping, Optional, Tuple
 import numpy as np
 import torch
 
-_ALIASES: Dict[str, str] = {
-    "stiff": "stiffness", "stiffness": "stiffness", "e": "stiffness", "young": "stiffness",
-    "youngs": "stiffness", "shear": "shear", "g": "shear", "thermal": "thermal", "kappa": "thermal",
-    "conductivity": "thermal", "heat": "thermal", "bulk": "bulk", "k": "bulk", "bulk_modulus": "bulk",
+# Accepted names of each term (objective.py:65-82 lists the reference's; "bulk" and its
+# names are this build's extension).
+_NAMES = {
+    "stiffness": ("stiffness", "stiff", "e", "young", "youngs"),
+    "shear": ("shear", "g"),
+    "thermal": ("thermal", "kappa", "conductivity", "heat"),
+    "bulk": ("bulk", "k", "bulk_modulus"),
 }
-_CANONICAL = ("stiffness", "shear", "thermal", "bulk")
+_TERM_OF = {alias: term for term, names in _NAMES.items() for alias in names}
+
+# One weighted term: "name", "name:w", "name=w", "w*name" or "name*w".
+_NUM = r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?"
+_WORD = r"[^\s:=*,+]+"
+_TERM_RE = re.compile(rf"^(?:(?P<w1>{_NUM})\*(?P<n1>{_WORD})|(?P<n2>{_WORD})\*(?P<w2>{_NUM})|"
+                      rf"(?P<n3>{_WORD})(?:[:=](?P<w3>{_NUM}))?)$")
 
 
-def _canon(name: str) -> str:
-    n = str(name).strip().lower()
-    n = _ALIASES.get(n, n)
-    if n not in _CANONICAL:
-        raise ValueError(f"Unknown objective term: {name!r}. Supported: {list(_CANONICAL)}")
-    return n
+def _term(name: str) -> str:
+    key = str(name).strip().lower()
+    if key not in _TERM_OF:
+        raise ValueError(f"objective term {name!r} is not one of {sorted(_NAMES)} (or their aliases)")
+    return _TERM_OF[key]
+
+
+def _is_number(text: str) -> bool:
+    return re.fullmatch(_NUM, text) is not None
 
 
 def parse_objective_spec(spec: Optional[str]) -> Dict[str, float]:
-    """'stiffness', 'stiffness:0.4,thermal:0.6', '0.4*stiffness+0.6*thermal', 'shear=0.3+thermal=0.7'
-    (objective.py:93-157); weights are used as given."""
-    if spec is None or not str(spec).strip():
+    """Weights of an objective spec (objective.py:93-157): a single term name, or terms
+    joined by ',' or '+', each "name", "name:w", "name=w", "w*name" or "name*w" (spaces
+    ignored; a term named twice adds up; weights are used as given).  An empty spec is
+    the reference's default, shear."""
+    text = "" if spec is None else str(spec).strip()
+    if not text:
         return {"shear": 1.0}
-    s = str(spec).strip()
-    try:
-        return {_canon(s): 1.0}
-    except ValueError:
-        pass
-    parts = [p for p in re.split(r"[,+]", s.replace(" ", "")) if p]
-    if not parts:
-        raise ValueError(f"Invalid objective spec: {spec!r}")
-
-    def num(x):
-        try:
-            float(x)
-            return True
-        except ValueError:
-            return False
-
-    out: Dict[str, float] = {}
-    for part in parts:
-        if ":" in part:
-            name, w = part.split(":", 1)
-            w = float(w)
-        elif "=" in part:
-            name, w = part.split("=", 1)
-            w = float(w)
-        elif "*" in part:
-            a, b = part.split("*", 1)
-            if num(a) and not num(b):
-                w, name = float(a), b
-            elif num(b) and not num(a):
-                w, name = float(b), a
-            else:
-                raise ValueError(f"Invalid term {part!r} in objective spec {spec!r}. "
-                                 f"Expected something like '0.4*stiffness' or 'stiffness:0.4'.")
-        else:
-            name, w = part, 1.0
-        t = _canon(name)
-        out[t] = out.get(t, 0.0) + float(w)
-    return out
+    if text.lower() in _TERM_OF:
+        return {_TERM_OF[text.lower()]: 1.0}
+    weights: Dict[str, float] = {}
+    pieces = [p for p in re.split(r"[,+]", text.replace(" ", "")) if p]
+    if not pieces:
+        raise ValueError(f"objective spec {spec!r} names no term")
+    for piece in pieces:
+        m = _TERM_RE.match(piece)
+        name = m and (m["n1"] or m["n2"] or m["n3"])
+        if not m or _is_number(name):
+            raise ValueError(f"cannot read {piece!r} of objective spec {spec!r}: write a term as "
+                             f"'name', 'name:w', 'name=w' or 'w*name'")
+        w = m["w1"] or m["w2"] or m["w3"]
+        term = _term(name)
+        weights[term] = weights.get(term, 0.0) + (float(w) if w is not None else 1.0)
+    return weights
 
 
 def objective_requires(spec: str) -> Tuple[bool, bool]:

@@ -101,8 +101,17 @@ def test_objective_grammar():
     assert parse_objective_spec("E:0.5, kappa:0.5") == {"stiffness": 0.5, "thermal": 0.5}
     assert parse_objective_spec(None) == {"shear": 1.0}
     assert objective_requires("bulk") == (True, False)
-    with pytest.raises(ValueError):
-        parse_objective_spec("strength")
+    # the reference's other forms (objective.py:93-157): name*w, repeated terms add up,
+    # a lone name with surrounding spaces, scientific weights
+    assert parse_objective_spec("stiffness*2,g:1") == {"stiffness": 2.0, "shear": 1.0}
+    assert parse_objective_spec("e+e") == {"stiffness": 2.0}
+    assert parse_objective_spec("  Thermal ") == {"thermal": 1.0}
+    assert parse_objective_spec("1e-3*shear") == {"shear": 1e-3}
+    # rejected as the reference rejects them: unknown names, two numbers or two names
+    # around '*', a bare weight, an exponent with '+' (the separator splits it)
+    for bad in ("strength", "2*3", "e*g", "0.5", "stiffness:abc", "stiffness:1e+3", ","):
+        with pytest.raises(ValueError):
+            parse_objective_spec(bad)
 
 
 def test_workload_designs_match_oracle():
