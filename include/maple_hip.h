@@ -129,6 +129,22 @@ int maple_set_cg(maple_ctx* ctx, int flexible);
  * one after the other on the caller's stream (on = 0). */
 int maple_set_concurrency(maple_ctx* ctx, int on);
 
+/* Run the CG iterations of maple_homogenize as hipGraphs (on = 1) or as single
+ * launches (on = 0, the default).  Each iteration is two captured segments (the
+ * operator, step length, update and convergence check; the preconditioner, next
+ * direction) replayed on the caller's stream; the convergence poll and the launch
+ * tables of the live cells stay on the host between them, and a segment is captured
+ * again whenever what its launches were captured with changes (the live cells, the
+ * FP64 fallback, the islands, any setting).  Same arithmetic in the same order as the
+ * single launches: results are bitwise equal.  With graphs on, homogenize copies rho
+ * into the context first (one more device buffer of the level-0 size).  Kernel timing
+ * (maple_timing_enable) runs the iterations as single launches.  The reference
+ * launches every op of its V-cycle from Python (EBE_GMG.py:886-984); this has no
+ * counterpart there. */
+int maple_set_graphs(maple_ctx* ctx, int on);
+/* Segments captured and graph launches made so far by the context (either may be NULL). */
+int maple_graph_counts(const maple_ctx* ctx, long long* captures, long long* launches);
+
 /* MAPLE_PREC_MIXED only: a solve that has not converged after `iters` CG
  * iterations builds the FP64 hierarchy of the same density and finishes on the
  * FP64 V-cycle (0 = never; default 300).  An FP32 V-cycle cannot resolve the

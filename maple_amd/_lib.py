@@ -52,6 +52,8 @@ SIGNATURES = {
     "maple_set_cg": (_I, [_P, _I]),
     "maple_set_mixed_fallback": (_I, [_P, _I]),
     "maple_set_concurrency": (_I, [_P, _I]),
+    "maple_set_graphs": (_I, [_P, _I]),
+    "maple_graph_counts": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "maple_set_coarse_refresh": (_I, [_P, _I]),
     "maple_set_variant": (_I, [_P, _I, _I]),
     "maple_homogenize": (_I, [_P, _P, _I, _D, _I, _P, _P, _IP, _P]),

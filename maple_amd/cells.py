@@ -128,6 +128,16 @@ class CellBatch:
         """Solve elastic and thermal at the same time (second stream and host thread) or in turn."""
         L.check(self.lib.maple_set_concurrency(self._h, int(bool(on))))
 
+    def set_graphs(self, on: bool = True):
+        """Run the CG iterations as captured hipGraphs (bitwise the same results)."""
+        L.check(self.lib.maple_set_graphs(self._h, int(bool(on))))
+
+    def graph_counts(self):
+        """(segments captured, graph launches) so far."""
+        a, b = C.c_longlong(0), C.c_longlong(0)
+        L.check(self.lib.maple_graph_counts(self._h, C.byref(a), C.byref(b)))
+        return int(a.value), int(b.value)
+
     def set_sweeps(self, pre: int = 2, post: int = 2):
         """Jacobi sweeps before / after the coarse correction (unequal: use flexible CG)."""
         L.check(self.lib.maple_set_sweeps(self._h, int(pre), int(post)))

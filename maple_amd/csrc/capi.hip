@@ -11,6 +11,7 @@
 #include "../../include/maple_hip.h"
 
 #include <algorithm>
+#include <initializer_list>
 #include <type_traits>
 #include <cmath>
 #include <cstring>
@@ -65,6 +66,23 @@ struct VC {
   T* rho = nullptr;   // level-0 densities (the FP64 form uses the caller's array)
 };
 
+// CG's preconditioned residual: FP64 (FP64 V-cycle) or, in the mixed form, the FP32
+// V-cycle's own output, never widened (the CG reads it as FP32).
+struct ZVec {
+  double* d = nullptr;
+  float* f = nullptr;
+};
+
+// One captured segment of a CG iteration (hipGraph) and what the capture decided on the
+// host: the key of everything baked into its kernel arguments, the preconditioned
+// residual's buffer and the form of the fused update.
+struct SegGraph {
+  hipGraphExec_t exec = nullptr;
+  unsigned long long key = ~0ULL;
+  ZVec z;
+  bool r32_ready = false, rzo_done = false;
+};
+
 struct Phys {
   int nd = 0, nc = 0;
   double *Ke = nullptr, *Fe = nullptr, *X0 = nullptr, *G = nullptr;   // per cell on device
@@ -113,6 +131,12 @@ struct Phys {
   int* isl_valid = nullptr;
   long long setups = 0;    // hierarchy builds so far (drives the coarsest-inverse refresh)
   bool built[2] = {false, false};   // a hierarchy exists: [0] FP64, [1] FP32
+  // CG iterations as hipGraphs (maple_set_graphs): the two segments of an iteration for
+  // the full launch tables ([0]) and for the current compacted ones ([1]), captured on
+  // this physics' own stream (a capture cannot run on a null stream)
+  SegGraph ga[2], gb[2];
+  hipStream_t cap = nullptr;
+  unsigned long long epoch = 0;   // compactions so far in this solve (0: full tables)
 };
 
 struct Timer {
@@ -161,6 +185,10 @@ struct maple_ctx {
   bool cav_ready = false;
   std::vector<void*> cav_allocs;                // freed when the configuration changes
   bool timing = false;
+  int graphs = 0;            // capture the CG iterations as hipGraphs
+  double* rho_own = nullptr;   // graphs on: the density of the current homogenization (a fixed address)
+  unsigned long long gen = 0;   // bumped by every setting that changes an iteration's kernels
+  long long graph_captures = 0, graph_launches = 0;
   Timer timers[3];   // 0: FP64 elastic level-0 operator, 1: FP64 thermal, 2: FP32 elastic
   unsigned long long* work_d = nullptr;   // per timer: elements x load cases of the timed launches
   std::vector<void*> allocs;
@@ -572,13 +600,6 @@ double* precondition(maple_ctx* c, Phys& P, const double* r, bool use64, hipStre
   return z;
 }
 
-// CG's preconditioned residual: FP64 (FP64 V-cycle) or, in the mixed form, the FP32
-// V-cycle's own output, never widened (the CG reads it as FP32).
-struct ZVec {
-  double* d = nullptr;
-  float* f = nullptr;
-};
-
 // r32_ready: the FP32 copy of r is already in the V-cycle's input (the mixed update wrote it)
 ZVec precondition_cg(maple_ctx* c, Phys& P, const double* r, bool use64, bool r32_ready, hipStream_t s) {
   ZVec z;
@@ -642,6 +663,7 @@ void full_launches(maple_ctx* c, Phys& P) {
   P.lm.done = P.done_d;
   P.lm.ndc = P.nc;
   P.nlive = c->ncells;
+  P.epoch = 0;
 }
 
 // Once cells have converged (host flags of the last poll), the iteration's launches
@@ -684,6 +706,7 @@ int compact_launches(maple_ctx* c, Phys& P, const int* done_h, hipStream_t s) {
   TRY(build(c->lv[0], c->lv[0].cnt_quad, P.ccq, P.cbq, P.lq));
   if (c->mfma_ok) TRY(build(c->lv[0], c->lv[0].cnt_mfma, P.ccm, P.cbm, P.lm));
   P.nlive = n;
+  P.epoch += 1;   // the iteration's captured launches no longer apply
   return 0;
 }
 
@@ -700,6 +723,45 @@ int enqueue_poll(maple_ctx* c, Phys& P, int slot, hipStream_t s) {
   const int n = c->ncells * P.nc;
   HIPCHK(hipMemcpyAsync(P.done_h + slot * (n + 1), P.done_d, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, s));
   HIPCHK(hipEventRecord(P.poll[slot], s));
+  return 0;
+}
+
+unsigned long long graph_key(std::initializer_list<unsigned long long> parts) {
+  unsigned long long h = 1469598103934665603ULL;
+  for (unsigned long long v : parts) h = (h ^ v) * 1099511628211ULL + (h >> 29);
+  return h;
+}
+
+// Runs one CG segment: eagerly on s, or (graphs on, timing off) as the hipGraph captured
+// for this key, capturing it first on the physics' own stream.  The body records its host
+// decisions in G, which a replay reuses.
+template <typename Body>
+int run_segment(maple_ctx* c, Phys& P, SegGraph& G, unsigned long long key, hipStream_t s, Body&& body) {
+  if (!c->graphs || c->timing) {
+    body(s, G);
+    return 0;
+  }
+  if (G.key != key || !G.exec) {
+    if (G.exec) (void)hipGraphExecDestroy(G.exec);
+    G.exec = nullptr;
+    G.key = ~0ULL;
+    if (!P.cap) HIPCHK(hipStreamCreateWithFlags(&P.cap, hipStreamNonBlocking));
+    HIPCHK(hipStreamBeginCapture(P.cap, hipStreamCaptureModeThreadLocal));
+    body(P.cap, G);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(P.cap, &g);
+    if (e != hipSuccess || !g) return fail(MAPLE_EDEVICE, std::string("CG iteration capture failed: ") + hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+      G.exec = nullptr;
+      return fail(MAPLE_EDEVICE, std::string("CG iteration graph instantiation failed: ") + hipGetErrorString(e2));
+    }
+    G.key = key;
+    __atomic_add_fetch(&c->graph_captures, 1, __ATOMIC_RELAXED);   // the two physics' threads
+  }
+  HIPCHK(hipGraphLaunch(G.exec, s));
+  __atomic_add_fetch(&c->graph_launches, 1, __ATOMIC_RELAXED);
   return 0;
 }
 
@@ -741,30 +803,52 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   TRY(enqueue_poll(c, P, 0, s));
   bool all = false;
   TRY(poll_done(c, P, 0, &all));
-  for (int it = 0; it < maxit && !all; ++it) {
-    apply_fine64(c, P, 1, rho, P.p, nullptr, P.q, P.partial, s);
-    reduce_partials(op_launch(c, P, false), P.nc, P.partial, P.sc, SC_PQ, s);
-    pcg_alpha(c->ncells, P.nc, P.sc, P.done_d, P.done_d + c->ncells * P.nc, s);
+  // An iteration is two segments: (A) q = K p, alpha, the update of u and r, the
+  // convergence check; (B) the preconditioner, beta and the next direction.  With
+  // graphs on, each runs as a captured hipGraph whose key covers everything baked into
+  // its launches; the convergence poll and the launch compaction stay on the host between.
+  auto segA = [&](hipStream_t st, SegGraph& G) {
+    apply_fine64(c, P, 1, rho, P.p, nullptr, P.q, P.partial, st);
+    reduce_partials(op_launch(c, P, false), P.nc, P.partial, P.sc, SC_PQ, st);
+    pcg_alpha(c->ncells, P.nc, P.sc, P.done_d, P.done_d + c->ncells * P.nc, st);
     // mixed form: the update also writes the FP32 preconditioner's input and, from
     // the previous FP32 z, the flexible dot (r_new, z_old)
-    bool r32_ready = false, rzo_done = false;
+    G.r32_ready = G.rzo_done = false;
     if (!use64 && pcg_update_ur_mixed(P.ln[0], P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, P.v32.B[0],
-                                      flex ? z.f : nullptr, P.partial, partial2, s) == 0) {
-      r32_ready = true;
-      rzo_done = flex && z.f != nullptr;
+                                      flex ? z.f : nullptr, P.partial, partial2, st) == 0) {
+      G.r32_ready = true;
+      G.rzo_done = flex && z.f != nullptr;
     } else {
-      pcg_update_ur(P.ln[0], P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, P.partial, s);
+      pcg_update_ur(P.ln[0], P.nd, P.nc, P.sc, P.p, P.q, P.u, P.r, P.partial, st);
     }
-    reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RR_LIVE, s);
-    if (rzo_done) reduce_partials(P.ln[0], P.nc, partial2, P.sc, SC_RZO, s);
-    pcg_check(c->ncells, P.nc, P.sc, P.done_d, s);
+    reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RR_LIVE, st);
+    if (G.rzo_done) reduce_partials(P.ln[0], P.nc, partial2, P.sc, SC_RZO, st);
+    pcg_check(c->ncells, P.nc, P.sc, P.done_d, st);
+  };
+  bool r32_ready = false, rzo_done = false;
+  auto segB = [&](hipStream_t st, SegGraph& G) {
+    if (flex && !rzo_done) {
+      dot_rz(P, P.r, z, P.partial, st);
+      reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RZO, st);
+    }
+    G.z = precondition_cg(c, P, P.r, use64, r32_ready && !use64, st);
+    dot_rz_mean(c, P, G.z, st);
+    pcg_beta(c->ncells, P.nc, flex, P.sc, st);
+    update_p(c, P, G.z, st);
+  };
+  for (int it = 0; it < maxit && !all; ++it) {
+    const int gs = P.epoch == 0 ? 0 : 1;
+    const unsigned long long kbase = graph_key({P.epoch, (unsigned long long)P.nlive, (unsigned long long)use64,
+                                                (unsigned long long)flex, (unsigned long long)c->nisl,
+                                                (unsigned long long)(uintptr_t)rho, c->gen});
+    SegGraph& GA = P.ga[gs];
+    TRY(run_segment(c, P, GA, graph_key({kbase, (unsigned long long)(uintptr_t)z.f,
+                                         (unsigned long long)(uintptr_t)z.d}), s, segA));
+    r32_ready = GA.r32_ready;
+    rzo_done = GA.rzo_done;
     const int slot = (it + 1) & 1;
     TRY(enqueue_poll(c, P, slot, s));
     if (it + 1 < maxit) {
-      if (flex && !rzo_done) {
-        dot_rz(P, P.r, z, P.partial, s);
-        reduce_partials(P.ln[0], P.nc, P.partial, P.sc, SC_RZO, s);
-      }
       // An FP32 V-cycle cannot resolve the soft modes of solid islands held by 1e-9
       // voids: a mixed solve still running after `fallback` iterations finishes on
       // the FP64 hierarchy of the same density (flexible CG absorbs the switch).
@@ -774,10 +858,11 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
         setup_fp64(c, P, rho, s, true);
         use64 = true;
       }
-      z = precondition_cg(c, P, P.r, use64, r32_ready && !use64, s);
-      dot_rz_mean(c, P, z, s);
-      pcg_beta(c->ncells, P.nc, flex, P.sc, s);
-      update_p(c, P, z, s);
+      SegGraph& GB = P.gb[gs];
+      TRY(run_segment(c, P, GB, graph_key({kbase, (unsigned long long)use64, (unsigned long long)r32_ready,
+                                           (unsigned long long)rzo_done, (unsigned long long)(uintptr_t)z.f,
+                                           (unsigned long long)(uintptr_t)z.d}), s, segB));
+      z = GB.z;
     }
     // the device is already working on the next iteration while we look at this one
     TRY(poll_done(c, P, slot, &all));
@@ -1095,9 +1180,13 @@ int maple_ctx_destroy(maple_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   for (void* p : c->cav_allocs) (void)hipFree(p);
   for (void* p : c->host_allocs) (void)hipHostFree(p);
-  for (Phys& P : c->ph)
+  for (Phys& P : c->ph) {
     for (int k = 0; k < 2; ++k)
       if (P.poll[k]) (void)hipEventDestroy(P.poll[k]);
+    for (SegGraph* G : {&P.ga[0], &P.ga[1], &P.gb[0], &P.gb[1]})
+      if (G->exec) (void)hipGraphExecDestroy(G->exec);
+    if (P.cap) (void)hipStreamDestroy(P.cap);
+  }
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->join) (void)hipEventDestroy(c->join);
   if (c->side) (void)hipStreamDestroy(c->side);
@@ -1127,6 +1216,7 @@ int maple_set_element_matrices(maple_ctx* c, int cell, const double* Ke24, const
 
 int maple_set_solver(maple_ctx* c, double omega, int nsmooth, int warm_start) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
+  c->gen += 1;   // captured CG iterations of the old setting no longer apply
   if (!(omega > 0.0 && omega < 2.0)) return fail(MAPLE_EINVAL, "omega must lie in (0, 2)");
   if (nsmooth < 1 || nsmooth > 16) return fail(MAPLE_EINVAL, "nsmooth must lie in [1, 16]");
   c->omega = omega;
@@ -1138,6 +1228,7 @@ int maple_set_solver(maple_ctx* c, double omega, int nsmooth, int warm_start) {
 
 int maple_set_variant(maple_ctx* c, int kernel, int variant) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
+  c->gen += 1;   // captured CG iterations of the old setting no longer apply
   if (kernel == MAPLE_KERNEL_FINE_OPERATOR) {
     if (variant < 0 || variant > 9) return fail(MAPLE_EINVAL, "unknown variant");
     c->lv[0].fine.var = variant;
@@ -1183,6 +1274,7 @@ int maple_set_mixed_fallback(maple_ctx* c, int iters) {
 
 int maple_set_cg(maple_ctx* c, int flexible) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
+  c->gen += 1;   // captured CG iterations of the old setting no longer apply
   if (flexible < -1 || flexible > 2) return fail(MAPLE_EINVAL, "CG variant must be -1, 0, 1 or 2");
   c->flexible = flexible;
   return 0;
@@ -1190,6 +1282,7 @@ int maple_set_cg(maple_ctx* c, int flexible) {
 
 int maple_set_islands(maple_ctx* c, int on, double threshold) {
   TRY(check_ctx(c));
+  c->gen += 1;
   if (!(threshold > 0.0)) return fail(MAPLE_EINVAL, "island threshold must be positive");
   c->isl_thr = threshold;
   c->islands = false;
@@ -1225,8 +1318,26 @@ int maple_set_islands(maple_ctx* c, int on, double threshold) {
 
 int maple_num_islands(const maple_ctx* c) { return c ? c->nisl : 0; }
 
+int maple_set_graphs(maple_ctx* c, int on) {
+  TRY(check_ctx(c));
+  if (on && !c->rho_own) {
+    TRY(set_device(c));
+    TRY(dalloc(c, &c->rho_own, (size_t)c->lv[0].Ntot));
+  }
+  c->graphs = on ? 1 : 0;
+  return 0;
+}
+
+int maple_graph_counts(const maple_ctx* c, long long* captures, long long* launches) {
+  TRY(check_ctx(c));
+  if (captures) *captures = __atomic_load_n(&c->graph_captures, __ATOMIC_RELAXED);
+  if (launches) *launches = __atomic_load_n(&c->graph_launches, __ATOMIC_RELAXED);
+  return 0;
+}
+
 int maple_set_sweeps(maple_ctx* c, int pre, int post) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
+  c->gen += 1;   // captured CG iterations of the old setting no longer apply
   if (pre < 1 || pre > 16 || post < 0 || post > 16) return fail(MAPLE_EINVAL, "sweeps: pre in [1, 16], post in [0, 16]");
   c->nsmooth = pre;
   c->npost = post;
@@ -1235,6 +1346,7 @@ int maple_set_sweeps(maple_ctx* c, int pre, int post) {
 
 int maple_set_smoother(maple_ctx* c, int kind) {
   if (!c) return fail(MAPLE_EINVAL, "null context");
+  c->gen += 1;   // captured CG iterations of the old setting no longer apply
   if (kind != MAPLE_SMOOTH_JACOBI && kind != MAPLE_SMOOTH_L1_JACOBI) return fail(MAPLE_EINVAL, "unknown smoother");
   c->smoother = kind;
   return 0;
@@ -1247,6 +1359,11 @@ int maple_homogenize(maple_ctx* c, const double* rho, int physics, double rtol, 
   if (!(physics & (MAPLE_ELASTIC | MAPLE_THERMAL))) return fail(MAPLE_EINVAL, "no physics selected");
   if (!(rtol > 0.0) || maxit < 1) return fail(MAPLE_EINVAL, "need rtol > 0 and maxit >= 1");
   hipStream_t s = (hipStream_t)stream;
+  if (c->graphs) {
+    // the captured CG iterations read the density at a fixed address
+    HIPCHK(hipMemcpyAsync(c->rho_own, rho, sizeof(double) * c->lv[0].Ntot, hipMemcpyDeviceToDevice, s));
+    rho = c->rho_own;
+  }
   TRY(find_islands(c, rho, s));
   const bool both = (physics & MAPLE_ELASTIC) && (physics & MAPLE_THERMAL);
   if (!both || !c->concurrent) {
