@@ -306,7 +306,7 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
     for (int l = 0; l < L; ++l) {
       const long long N = c->lv[l].Ntot;
       const size_t nv = (size_t)nc * nd * N;
-      if (l >= 1) TRY(dalloc(c, (TT**)&V.S[l], (size_t)27 * nd * nd * N));
+      if (l >= 1) TRY(dalloc(c, (TT**)&V.S[l], (size_t)14 * nd * nd * N));   // symmetric half
       if (l < L - 1) {
         TRY(dalloc(c, (TT**)&V.dinv[l], (size_t)nd * N));
         TRY(dalloc(c, (TT**)&V.R[l], nv));
@@ -1661,10 +1661,10 @@ int maple_mg_stencil(maple_ctx* c, int physics, int prec, int level, double* S, 
   TRY(mg_built(*P, prec));
   if (level < 1 || level >= c->nlevels) return fail(MAPLE_EINVAL, "stencils exist on levels 1 .. nlevels-1");
   if (!S) return fail(MAPLE_EINVAL, "null output");
-  const long long n = 27LL * P->nd * P->nd * c->lv[level].Ntot;
   hipStream_t s = (hipStream_t)stream;
-  if (prec == MAPLE_PREC_FP64) vec_copy<double, double>(n, P->v64.S[level], S, s);
-  else vec_copy<double, float>(n, P->v32.S[level], S, s);
+  // all 27 blocks, the lower ones from the stored half's transposes
+  if (prec == MAPLE_PREC_FP64) stencil_expand<double>(c->lv[level].node, P->nd, P->v64.S[level], S, s);
+  else stencil_expand<float>(c->lv[level].node, P->nd, P->v32.S[level], S, s);
   HIPCHK(hipGetLastError());
   return 0;
 }

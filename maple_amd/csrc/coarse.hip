@@ -1,5 +1,10 @@
 // Coarse multigrid levels on assembled 27-point block stencils.
 //
+// Storage is the symmetric half: the operator is symmetric, K(n, m) = K(m, n)^T, so each
+// node keeps the 14 blocks of the offsets d = 13..26 ((dx, dy, dz) lexicographically
+// >= 0, the centre first) and block d < 13 of node n is read, transposed, as block
+// 26 - d of its neighbour n + off(d): 14 instead of 27 blocks stored and streamed.
+//
 // The reference keeps per-element coarse matrices and re-projects them with
 // the local trilinear operator (sparse_coarsen_ke, EBE_GMG.py:226-337), and
 // moves residuals with the periodic trilinear P / P^T (EBE_GMG.py:38-79,
@@ -38,6 +43,13 @@ __device__ __forceinline__ long long nid(const LevelCell& c, int x, int y, int z
 }
 
 __device__ __forceinline__ double w1(int t) { return t == 0 ? 1.0 : ((t == 1 || t == -1) ? 0.5 : 0.0); }
+
+// Plane of entry (i, j) of block d in the half storage, and whether it sits at the
+// neighbour n + off(d) (d < 13, transposed) instead of at the node itself.
+template <int ND>
+__device__ __forceinline__ int half_plane(int d, int i, int j) {
+  return d >= 13 ? (d - 13) * ND * ND + i * ND + j : (13 - d) * ND * ND + j * ND + i;
+}
 
 }  // namespace
 
@@ -81,7 +93,8 @@ __global__ __launch_bounds__(256) void k_stencil_apply(const LevelCell* __restri
         const unsigned nb = line + (unsigned)zs[dz];
         T srow[ND];
 #pragma unroll
-        for (int e = 0; e < ND; ++e) srow[e] = (S + (long long)(d * ND * ND + i * ND + e) * Ntot + c.off)[g32];
+        for (int e = 0; e < ND; ++e)
+          srow[e] = (S + (long long)half_plane<ND>(d, i, e) * Ntot + c.off)[d >= 13 ? g32 : nb];
 #pragma unroll
         for (int j = 0; j < NC; ++j)
 #pragma unroll
@@ -139,7 +152,8 @@ __global__ __launch_bounds__(256) void k_stencil_apply_node(const LevelCell* __r
 #pragma unroll
         for (int i = 0; i < ND; ++i)
 #pragma unroll
-          for (int e = 0; e < ND; ++e) sb[i][e] = (S + (long long)(d * ND * ND + i * ND + e) * Ntot + c.off)[g32];
+          for (int e = 0; e < ND; ++e)
+            sb[i][e] = (S + (long long)half_plane<ND>(d, i, e) * Ntot + c.off)[d >= 13 ? g32 : nb];
 #pragma unroll
         for (int j = 0; j < NC; ++j)
 #pragma unroll
@@ -199,10 +213,14 @@ __global__ __launch_bounds__(256) void k_stencil_dinv(const LevelCell* __restric
   for (int i = 0; i < nd; ++i) {
     double dv = 0.0;
     if (l1) {
-      for (int d = 0; d < 27; ++d)
-        for (int j = 0; j < nd; ++j) dv += fabs((double)S[(long long)(d * nd * nd + i * nd + j) * Ntot + gid]);
+      for (int d = 0; d < 27; ++d) {
+        const long long m = nid(c, wrap(x + d / 9 - 1, c.nx), wrap(y + (d / 3) % 3 - 1, c.ny), wrap(z + d % 3 - 1, c.nz));
+        for (int j = 0; j < nd; ++j)
+          dv += fabs((double)(d >= 13 ? S[(long long)((d - 13) * nd * nd + i * nd + j) * Ntot + gid]
+                                      : S[(long long)((13 - d) * nd * nd + j * nd + i) * Ntot + m]));
+      }
     } else {
-      dv = (double)S[(long long)(13 * nd * nd + i * nd + i) * Ntot + gid];
+      dv = (double)S[(long long)(i * nd + i) * Ntot + gid];   // centre block: the first stored
     }
     dinv[(long long)i * Ntot + gid] = dv != 0.0 ? (T)(1.0 / dv) : T(1);
   }
@@ -215,7 +233,32 @@ void stencil_dinv(const Launch& L, int nd, bool l1, const T* S, T* dinv, hipStre
 template void stencil_dinv<double>(const Launch&, int, bool, const double*, double*, hipStream_t);
 template void stencil_dinv<float>(const Launch&, int, bool, const float*, float*, hipStream_t);
 
-// blockIdx.y = neighbour offset d; one thread per coarse node.
+// out: the full 27-block stencil; blockIdx.y = neighbour offset d.
+template <typename T>
+__global__ __launch_bounds__(256) void k_stencil_expand(const LevelCell* __restrict__ cells,
+                                                        const int* __restrict__ blk_start, int ncells,
+                                                        long long Ntot, int nd, const T* __restrict__ S,
+                                                        double* __restrict__ out) {
+  LevelCell c; int ci, x, y, z; long long gid;
+  if (!node_of(cells, blk_start, ncells, lbid(), c, ci, x, y, z, gid)) return;
+  const int d = blockIdx.y;
+  const long long m = nid(c, wrap(x + d / 9 - 1, c.nx), wrap(y + (d / 3) % 3 - 1, c.ny), wrap(z + d % 3 - 1, c.nz));
+  for (int i = 0; i < nd; ++i)
+    for (int j = 0; j < nd; ++j)
+      out[(long long)(d * nd * nd + i * nd + j) * Ntot + gid] =
+          (double)(d >= 13 ? S[(long long)((d - 13) * nd * nd + i * nd + j) * Ntot + gid]
+                           : S[(long long)((13 - d) * nd * nd + j * nd + i) * Ntot + m]);
+}
+
+template <typename T>
+void stencil_expand(const Launch& L, int nd, const T* S, double* out, hipStream_t s) {
+  dim3 grid(L.nblocks, 27);
+  k_stencil_expand<T><<<grid, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, nd, S, out);
+}
+template void stencil_expand<double>(const Launch&, int, const double*, double*, hipStream_t);
+template void stencil_expand<float>(const Launch&, int, const float*, double*, hipStream_t);
+
+// blockIdx.y = stored neighbour offset 13 + y; one thread per coarse node.
 template <int ND, typename T>
 __global__ __launch_bounds__(256) void k_stencil_level1(const LevelCell* __restrict__ ccells,
                                                         const int* __restrict__ blk_start, int ncells,
@@ -227,7 +270,7 @@ __global__ __launch_bounds__(256) void k_stencil_level1(const LevelCell* __restr
   int kc;
   if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid, &kc)) return;
   const LevelCell cf = fcells[kc];
-  const int d = blockIdx.y;
+  const int d = 13 + blockIdx.y;   // the stored half
   const int dd[3] = {d / 9 - 1, (d / 3) % 3 - 1, d % 3 - 1};
   double acc[ND][ND];
 #pragma unroll
@@ -256,13 +299,13 @@ __global__ __launch_bounds__(256) void k_stencil_level1(const LevelCell* __restr
 #pragma unroll
   for (int i = 0; i < ND; ++i)
 #pragma unroll
-    for (int j = 0; j < ND; ++j) S[(long long)(d * ND * ND + i * ND + j) * Ntot_c + gid] = (T)acc[i][j];
+    for (int j = 0; j < ND; ++j) S[(long long)((d - 13) * ND * ND + i * ND + j) * Ntot_c + gid] = (T)acc[i][j];
 }
 
 template <typename T>
 void stencil_level1(const Launch& Lc, const Launch& Lf, int nd, const double* G, const double* rho, T* S,
                     hipStream_t s) {
-  dim3 grid(Lc.nblocks, 27);
+  dim3 grid(Lc.nblocks, 14);
   if (nd == 3) k_stencil_level1<3, T><<<grid, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, G, rho, S);
   else k_stencil_level1<1, T><<<grid, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, G, rho, S);
 }
@@ -280,7 +323,7 @@ __global__ __launch_bounds__(256) void k_stencil_galerkin(const LevelCell* __res
   int kc;
   if (!node_of(ccells, blk_start, ncells, lbid(), c, ci, x, y, z, gid, &kc)) return;
   const LevelCell cf = fcells[kc];
-  const int D = blockIdx.y;
+  const int D = 13 + blockIdx.y;   // the stored half
   const int Dx = D / 9 - 1, Dy = (D / 3) % 3 - 1, Dz = D % 3 - 1;
   double acc[ND][ND];
 #pragma unroll
@@ -304,23 +347,24 @@ __global__ __launch_bounds__(256) void k_stencil_galerkin(const LevelCell* __res
               const int fz = wrap(2 * z + sz, cf.nz);
               const long long fn = cf.off + ((long long)fx * cf.ny + fy) * cf.nz + fz;
               const int d = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+              const long long at = d >= 13 ? fn : nid(cf, wrap(fx + dx, cf.nx), wrap(fy + dy, cf.ny), wrap(fz + dz, cf.nz));
 #pragma unroll
               for (int i = 0; i < ND; ++i)
 #pragma unroll
                 for (int j = 0; j < ND; ++j)
-                  acc[i][j] = fma(w, (double)Sf[(long long)(d * ND * ND + i * ND + j) * Ntot_f + fn], acc[i][j]);
+                  acc[i][j] = fma(w, (double)Sf[(long long)half_plane<ND>(d, i, j) * Ntot_f + at], acc[i][j]);
             }
         }
     }
 #pragma unroll
   for (int i = 0; i < ND; ++i)
 #pragma unroll
-    for (int j = 0; j < ND; ++j) Sc[(long long)(D * ND * ND + i * ND + j) * Ntot_c + gid] = (T)acc[i][j];
+    for (int j = 0; j < ND; ++j) Sc[(long long)((D - 13) * ND * ND + i * ND + j) * Ntot_c + gid] = (T)acc[i][j];
 }
 
 template <typename T>
 void stencil_galerkin(const Launch& Lc, const Launch& Lf, int nd, const T* Sf, T* Sc, hipStream_t s) {
-  dim3 grid(Lc.nblocks, 27);
+  dim3 grid(Lc.nblocks, 14);
   if (nd == 3) k_stencil_galerkin<3, T><<<grid, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, Lf.Ntot, Sf, Sc);
   else k_stencil_galerkin<1, T><<<grid, 256, 0, s>>>(Lc.cells, Lc.blk_start, Lc.ncells, Lc.Ntot, Lf.cells, Lf.Ntot, Sf, Sc);
 }

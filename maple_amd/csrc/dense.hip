@@ -24,13 +24,14 @@ __global__ __launch_bounds__(256) void k_dense_gamma(const LevelCell* __restrict
   const long long nn = (long long)c.nx * c.ny * c.nz;
   double v[1] = {0.0};
   for (long long n = threadIdx.x; n < nn; n += blockDim.x)
-    for (int i = 0; i < nd; ++i) v[0] += (double)S[(long long)(13 * nd * nd + i * nd + i) * Ntot + c.off + n];
+    for (int i = 0; i < nd; ++i) v[0] += (double)S[(long long)(i * nd + i) * Ntot + c.off + n];   // centre block
   __shared__ double tot;
   block_sum<1>(v, &tot);
   if (threadIdx.x == 0) gamma[ci] = tot / (double)dcells[ci].n;
 }
 
-// A[(n,i),(m,j)] = sum_{d: n+d == m} S[d](i,j)(n) + gamma/N delta_ij
+// A[(n,i),(m,j)] = sum_{d: n+d == m} S[d](i,j)(n) + gamma/N delta_ij; S in the half storage
+// (blocks d >= 13 at the node, d < 13 transposed at the neighbour m)
 template <typename T>
 __global__ __launch_bounds__(256) void k_dense_assemble(const LevelCell* __restrict__ cells,
                                                         const DenseCell* __restrict__ dcells, long long Ntot,
@@ -56,7 +57,8 @@ __global__ __launch_bounds__(256) void k_dense_assemble(const LevelCell* __restr
       for (int dz = -1; dz <= 1; ++dz) {
         if (wrap(zr + dz, c.nz) != zc) continue;
         const int d = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
-        acc += (double)S[(long long)(d * nd * nd + ir * nd + jc) * Ntot + c.off + nr];
+        acc += d >= 13 ? (double)S[(long long)((d - 13) * nd * nd + ir * nd + jc) * Ntot + c.off + nr]
+                       : (double)S[(long long)((13 - d) * nd * nd + jc * nd + ir) * Ntot + c.off + mc];
       }
     }
   }

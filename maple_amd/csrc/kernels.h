@@ -87,13 +87,17 @@ template <typename T>
 void fine_dinv(const Launch& L, int nd, bool l1, const double* Ke, const double* rho, T* dinv, hipStream_t s);
 
 // ---- coarse levels (coarse.hip) --------------------------------------------
-// Stencil S of a level: S[(d*ND*ND + i*ND + j)*Ntot + n] is the (i,j) entry of the
-// block coupling node n to node n+off(d), d = (dx+1)*9 + (dy+1)*3 + (dz+1).
+// Stencil S of a level, symmetric half: S[((d-13)*ND*ND + i*ND + j)*Ntot + n] is the (i,j)
+// entry of the block coupling node n to node n+off(d), d = (dx+1)*9 + (dy+1)*3 + (dz+1),
+// for d = 13..26; block d < 13 of node n is the transpose of block 26-d of n+off(d).
 template <int ND, typename T>
 void stencil_apply(const Launch& L, int mode, int ncase, const T* S, const T* u, const T* f,
                    const T* dinv, T omega, T* out, double* partial, hipStream_t s);
 template <typename T>
 void stencil_dinv(const Launch& L, int nd, bool l1, const T* S, T* dinv, hipStream_t s);
+// All 27 blocks of a half-stored stencil, out[(d*nd*nd + i*nd + j)*Ntot + n] (FP64).
+template <typename T>
+void stencil_expand(const Launch& L, int nd, const T* S, double* out, hipStream_t s);
 // level-1 stencil straight from the voxel densities: block(N, d) =
 // sum over coarse elements E containing N and N+d of sum_c rho_child(E,c) G_c[..]
 template <typename T>
