@@ -25,6 +25,10 @@ Forms (one JSON line; ``value`` is the headline form):
                iterate's fields and those fields plus the change the learned
                network predicts between the two densities.
                value = cells x iterations / s.
+               The V-cycle smooths with 3+3 l1-Jacobi sweeps whose weights are the
+               reciprocal Chebyshev nodes of [0.1, 1] (maple_set_chebyshev).
+* ``plain-cheb`` the plain form with the headline's smoothing (FP64, one cell, zero
+               start): ``speedup_vs_plain_fp64_same_smoother`` is the headline over it.
 * ``headline-lockstep`` the same 16 cells as one batch (its level-0 operator
                launches do not overlap another group's, so their HIP-event spans
                are their durations: the roofline is read there).
@@ -97,7 +101,8 @@ def _x0(ncells):
 
 def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", warm: bool = False,
              learned: bool = False, block: bool = False, islands: bool = False, groups: int = 1, refresh: int = 1,
-             maxit: int = 5000, profile_kernel: bool = True, tune: bool = True):
+             maxit: int = 5000, profile_kernel: bool = True, tune: bool = True, cheb: float = 0.0, sweeps: int = 2,
+             graphs: bool = False):
     """groups > 1: the cells advance in that many concurrent groups (maple_amd.concurrent),
     each its own batch, stream and host thread; every cell's trajectory is unchanged."""
     import numpy as np
@@ -116,6 +121,12 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
             cells = maple_amd.CellBatch([R] * per, nlevels=NLEVELS, device=dev, warm_start=warm,
                                         precision=precision, block_cg=block, islands=islands)
             cells.set_coarse_refresh(refresh)
+            if sweeps != 2:
+                cells.set_sweeps(sweeps, sweeps)
+            if cheb > 0.0:
+                cells.set_chebyshev(cheb)
+            if graphs:
+                cells.set_graphs(True)
             if tune:
                 from maple_amd import tune as mt
                 tuned.update({k: v for k, (v, _) in mt.autotune(cells).items()})
@@ -226,6 +237,8 @@ def run_config4(steps: int, precision: str = "mixed", maxit: int = 3000):
     cells = maple_amd.CellBatch([c["r"] for c in spec], nlevels=NLEVELS, device=dev, precision=precision,
                                 warm_start=True, islands=True)
     cells.set_coarse_refresh(3)
+    cells.set_sweeps(3, 3)
+    cells.set_chebyshev(0.1)
     from maple_amd import tune as mt
     tuned = {k: v for k, (v, _) in mt.autotune(cells).items()}
     drv = maple_amd.OCDriver(cells, [c["weights"] for c in spec], [c["volfrac"] for c in spec], move=0.2, xmin=1e-3,
@@ -245,7 +258,8 @@ def run_config4(steps: int, precision: str = "mixed", maxit: int = 3000):
     return {"steps": steps, "ms_per_step": dt * 1e3 / steps, "cells": len(spec), "elements": elems,
             "cell_oc_iters_per_s": len(spec) * steps / dt, "equiv_64cube_oc_iters_per_s": elems / R ** 3 * steps / dt,
             "unconverged_solves": unconv, "max_cg_iters": worst, "tuned_variants": tuned,
-            "form": "one batch, mixed precision, warm start, island correction, coarsest inverse every 3rd density"}
+            "form": "one batch, mixed precision, warm start, island correction, coarsest inverse every 3rd density, "
+                    "3+3 Chebyshev-weighted l1-Jacobi sweeps over [0.1, 1]"}
 
 
 def run_config3(steps: int):
@@ -379,9 +393,11 @@ def cpu_baseline(n_iter: int, warmup: int = 2):
 FORMS = {
     "plain": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False, islands=False),
     "headline": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True, groups=4,
-                     refresh=3),
+                     refresh=3, cheb=0.1, sweeps=3),
     "headline-lockstep": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True,
-                              refresh=3),
+                              refresh=3, cheb=0.1, sweeps=3),
+    "plain-cheb": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False, islands=False, cheb=0.1,
+                       sweeps=3),
     "headline-block": dict(ncells=16, precision="mixed", warm=True, learned=True, block=True, islands=True,
                            groups=4, refresh=3),
     "fp64-warm": dict(ncells=1, precision="fp64", warm=True, learned=False, block=False, islands=False),
@@ -404,7 +420,8 @@ def main():
     ap.add_argument("--steps", type=int, default=TRAJ,
                     help="kept for the harness; every form times one whole 100-iteration trajectory")
     ap.add_argument("--warmup", type=int, default=2, help="untimed OC iterations on separate contexts first")
-    ap.add_argument("--forms", default="plain,headline,headline-lockstep", help="comma list of " + ", ".join(FORMS))
+    ap.add_argument("--forms", default="plain,plain-cheb,headline,headline-lockstep",
+                    help="comma list of " + ", ".join(FORMS))
     ap.add_argument("--no-config4", action="store_true", help="skip the 64-cell mixed-resolution batch")
     ap.add_argument("--no-config3", action="store_true", help="skip the per-iterate mixed-form error report")
     ap.add_argument("--no-config1", action="store_true", help="skip the 32^3 config-1 GPU run")
@@ -448,6 +465,8 @@ def main():
                    "form": head, "cells": spec["ncells"], "precision": spec["precision"],
                    "warm_start": spec["warm"], "learned_start": spec["learned"], "block_cg": spec["block"],
                    "island_correction": spec["islands"], "groups": spec.get("groups", 1),
+                   "smoother": (f"l1-Jacobi, {spec.get('sweeps', 2)}+{spec.get('sweeps', 2)} sweeps, Chebyshev weights "
+                                f"over [{spec['cheb']}, 1]") if spec.get("cheb") else "l1-Jacobi, 2+2 sweeps, omega 1.6",
                    "coarsest_inverse_every": spec.get("refresh", 1),
                    "resolution": R, "oc_iterations_timed": TRAJ,
                    "levels": NLEVELS, "rtol": 1e-6},
@@ -458,6 +477,10 @@ def main():
     }
     if "plain" in res and head != "plain":
         line["speedup_vs_plain_fp64"] = g["value"] / res["plain"]["value"]
+    if "plain-cheb" in res and head != "plain-cheb":
+        # the same preconditioner smoothing on the plain FP64 single cell: what the three
+        # chosen forms (mixed precision, batching, learned start) add on top of it
+        line["speedup_vs_plain_fp64_same_smoother"] = g["value"] / res["plain-cheb"]["value"]
     # roofline of the dominant kernel: from the lockstep run of the same workload when it
     # was run (its launches do not overlap another group's, so a launch's HIP-event span
     # is its own duration), else from the headline run itself

@@ -87,6 +87,16 @@ int maple_set_solver(maple_ctx* ctx, double omega, int nsmooth, int warm_start);
  * uses smooth_iter per level (EBE_GMG.py:886-984). */
 int maple_set_sweeps(maple_ctx* ctx, int pre, int post);
 
+/* Weights of the smoother sweeps: with lo = 0 (the default) every sweep takes the
+ * solver's omega; with 0 < lo < 1 the k sweeps of each side take the reciprocals of the
+ * k Chebyshev nodes of [lo, 1], which damp the error over that part of the l1-Jacobi
+ * spectrum (contained in (0, 1]) by the degree-k Chebyshev polynomial instead of
+ * (1 - omega lambda)^k, for the same operator applications.  Pre-sweeps use the weights
+ * in rising, post-sweeps in falling order; with equal counts the V-cycle stays
+ * symmetric.  Needs the l1-Jacobi smoother.  The reference smooths with 8-colour
+ * Gauss-Seidel or weighted Jacobi (EBE_GMG.py:395-459); this has no counterpart there. */
+int maple_set_chebyshev(maple_ctx* ctx, double lo);
+
 /* Smoother of the V-cycle: MAPLE_SMOOTH_JACOBI (weighted point Jacobi; the
  * reference's smooth_jacobi, EBE_GMG.py:395-411) or MAPLE_SMOOTH_L1_JACOBI
  * (the diagonal replaced by the absolute row sum, convergent for every

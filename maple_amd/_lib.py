@@ -53,6 +53,7 @@ SIGNATURES = {
     "maple_set_mixed_fallback": (_I, [_P, _I]),
     "maple_set_concurrency": (_I, [_P, _I]),
     "maple_set_graphs": (_I, [_P, _I]),
+    "maple_set_chebyshev": (_I, [_P, C.c_double]),
     "maple_graph_counts": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "maple_set_coarse_refresh": (_I, [_P, _I]),
     "maple_set_variant": (_I, [_P, _I, _I]),

@@ -142,6 +142,10 @@ class CellBatch:
         """Jacobi sweeps before / after the coarse correction (unequal: use flexible CG)."""
         L.check(self.lib.maple_set_sweeps(self._h, int(pre), int(post)))
 
+    def set_chebyshev(self, lo: float = 0.0):
+        """Smoother sweep weights from the Chebyshev nodes of [lo, 1] (0: every sweep omega)."""
+        L.check(self.lib.maple_set_chebyshev(self._h, float(lo)))
+
     def set_islands(self, on: bool = True, threshold: float = 1e-4):
         """Island correction of the preconditioner: small solid islands (rho > threshold,
         at most 16 elements and 32 nodes) are solved exactly beside the V-cycle."""

@@ -158,6 +158,8 @@ struct maple_ctx {
   double omega = 1.6;   // l1-Jacobi: the spectrum of Dinv K lies in (0, 1], any omega < 2 converges
   int nsmooth = 2;          // Jacobi sweeps before the coarse correction (the first from zero)
   int npost = 2;            // and after it
+  double cheb_lo = 0.0;     // > 0: Chebyshev weights over [cheb_lo, 1] of the l1-Jacobi spectrum
+  std::vector<double> wpre, wpost;   // weight of each pre / post sweep
   int warm = 0;
   int smoother = MAPLE_SMOOTH_L1_JACOBI;
   int flexible = -1;   // -1: flexible CG in the mixed form only; 2: block CG
@@ -410,6 +412,28 @@ int alloc_phys(maple_ctx* c, Phys& P, int nd, int nc) {
 
 // ---- solver pieces ---------------------------------------------------------
 
+// Weights of the smoother sweeps: omega for every sweep, or, with the Chebyshev schedule
+// on, the reciprocals of the Chebyshev nodes of [lo, 1] (the l1-Jacobi spectrum of D^-1 K
+// lies in (0, 1]): k sweeps x <- x + w_i D^-1 (f - K x) then damp the error by the degree-k
+// Chebyshev polynomial of that interval, max |p| = 1 / T_k((1 + lo) / (1 - lo)), for the
+// same operator applications as k Jacobi sweeps.  Pre-sweeps take the weights in rising
+// order, post-sweeps in falling order; equal counts keep the V-cycle symmetric (the error
+// propagators are polynomials in D^-1 K).
+std::vector<double> sweep_weights(const maple_ctx* c, int k, bool rising) {
+  std::vector<double> w(std::max(k, 1), c->omega);
+  if (c->cheb_lo <= 0.0 || k < 1) return w;
+  const double a = c->cheb_lo, b = 1.0, pi = 3.14159265358979323846;
+  for (int i = 0; i < k; ++i) w[i] = 1.0 / (0.5 * (b + a) + 0.5 * (b - a) * std::cos((2.0 * i + 1.0) * pi / (2.0 * k)));
+  std::sort(w.begin(), w.end());
+  if (!rising) std::reverse(w.begin(), w.end());
+  return w;
+}
+
+void set_sweep_weights(maple_ctx* c) {
+  c->wpre = sweep_weights(c, c->nsmooth, true);
+  c->wpost = sweep_weights(c, c->npost, false);
+}
+
 void timed_begin(maple_ctx* c, int kind, hipStream_t s) {
   if (!c->timing) return;
   Timer& t = c->timers[kind];
@@ -485,7 +509,9 @@ void apply_fine64(maple_ctx* c, Phys& P, int mode, const double* rho, const doub
 
 // Operator of level l of hierarchy V (mode 2: residual, 3: Jacobi sweep).
 template <typename T>
-void apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const T* u, const T* f, T* out, hipStream_t s) {
+void apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const T* u, const T* f, T* out, hipStream_t s,
+             double w = -1.0) {
+  const T om = (T)(w > 0.0 ? w : c->omega);
   if (l == 0) {
     const int kind = P.nd == 3 ? (sizeof(T) == 8 ? 0 : 2) : (sizeof(T) == 8 ? 1 : -1);
     Launch L = op_launch(c, P, false, sizeof(T) == 4);
@@ -493,29 +519,29 @@ void apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const T* u, const
     if (kind >= 0) timed_begin(c, kind, s);
     if constexpr (sizeof(T) == 4) {
       if (P.nd == 3 && L.var == 9) {
-        fine_apply_mfma(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, s);
+        fine_apply_mfma(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], om, out, s);
         if (kind >= 0) timed_end(c, kind, s);
         return;
       }
     }
     if (P.nd == 3 && L.var == 8)
-      fine_apply_nodeb64<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+      fine_apply_nodeb64<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], om, out, nullptr, s);
     else if (P.nd == 3 && L.var >= 4)
-      fine_apply_nodeb<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+      fine_apply_nodeb<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], om, out, nullptr, s);
     else if (P.nd == 3 && L.var >= 2)
-      fine_apply_node<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+      fine_apply_node<T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], om, out, nullptr, s);
     else if (P.nd == 3)
-      fine_apply<3, T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+      fine_apply<3, T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], om, out, nullptr, s);
     else if (L.var >= 2)
-      thermal_apply64<T>(L, mode, P.nc, P.lap, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+      thermal_apply64<T>(L, mode, P.nc, P.lap, V.rho, u, f, V.dinv[0], om, out, nullptr, s);
     else
-      fine_apply<1, T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], (T)c->omega, out, nullptr, s);
+      fine_apply<1, T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], om, out, nullptr, s);
     if (kind >= 0) timed_end(c, kind, s);
   } else {
     if (P.nd == 3)
-      stencil_apply<3, T>(P.ln[l], mode, P.nc, V.S[l], u, f, V.dinv[l], (T)c->omega, out, nullptr, s);
+      stencil_apply<3, T>(P.ln[l], mode, P.nc, V.S[l], u, f, V.dinv[l], om, out, nullptr, s);
     else
-      stencil_apply<1, T>(P.ln[l], mode, P.nc, V.S[l], u, f, V.dinv[l], (T)c->omega, out, nullptr, s);
+      stencil_apply<1, T>(P.ln[l], mode, P.nc, V.S[l], u, f, V.dinv[l], om, out, nullptr, s);
   }
 }
 
@@ -566,9 +592,9 @@ T* vcycle_t(maple_ctx* c, Phys& P, VC<T>& V, int l, const T* b, hipStream_t s) {
   }
   T* x = V.X[l];
   T* t = V.W[l];
-  vec_scale_dinv<T>(P.ln[l], P.nd, P.nc, (T)c->omega, V.dinv[l], b, x, s);
+  vec_scale_dinv<T>(P.ln[l], P.nd, P.nc, (T)c->wpre[0], V.dinv[l], b, x, s);
   for (int k = 1; k < c->nsmooth; ++k) {
-    apply_t<T>(c, P, V, l, 3, x, b, t, s);
+    apply_t<T>(c, P, V, l, 3, x, b, t, s, c->wpre[k]);
     std::swap(x, t);
   }
   apply_t<T>(c, P, V, l, 2, x, b, V.R[l], s);
@@ -576,7 +602,7 @@ T* vcycle_t(maple_ctx* c, Phys& P, VC<T>& V, int l, const T* b, hipStream_t s) {
   T* e = vcycle_t<T>(c, P, V, l + 1, V.B[l + 1], s);
   prolong_add<T>(P.ln[l], P.ln[l + 1], P.nd, P.nc, e, x, s);
   for (int k = 0; k < c->npost; ++k) {
-    apply_t<T>(c, P, V, l, 3, x, b, t, s);
+    apply_t<T>(c, P, V, l, 3, x, b, t, s, c->wpost[k]);
     std::swap(x, t);
   }
   return x;
@@ -1081,6 +1107,7 @@ int maple_ctx_create(int device, int ncells, const int* dims_host, int nlevels, 
   c->dims.assign(dims_host, dims_host + 3 * ncells);
   c->all_nz64 = true;
   for (int i = 0; i < ncells; ++i) c->all_nz64 = c->all_nz64 && dims_host[3 * i + 2] == 64;
+  set_sweep_weights(c);
   int rc = set_device(c);
   auto bail = [&](int code) {
     std::string keep = g_err;
@@ -1223,6 +1250,7 @@ int maple_set_solver(maple_ctx* c, double omega, int nsmooth, int warm_start) {
   c->nsmooth = nsmooth;
   c->npost = nsmooth;
   c->warm = warm_start ? 1 : 0;
+  set_sweep_weights(c);
   return 0;
 }
 
@@ -1341,6 +1369,18 @@ int maple_set_sweeps(maple_ctx* c, int pre, int post) {
   if (pre < 1 || pre > 16 || post < 0 || post > 16) return fail(MAPLE_EINVAL, "sweeps: pre in [1, 16], post in [0, 16]");
   c->nsmooth = pre;
   c->npost = post;
+  set_sweep_weights(c);
+  return 0;
+}
+
+int maple_set_chebyshev(maple_ctx* c, double lo) {
+  if (!c) return fail(MAPLE_EINVAL, "null context");
+  if (!(lo >= 0.0 && lo < 1.0)) return fail(MAPLE_EINVAL, "Chebyshev interval: lo must lie in [0, 1)");
+  if (lo > 0.0 && c->smoother != MAPLE_SMOOTH_L1_JACOBI)
+    return fail(MAPLE_EINVAL, "the Chebyshev schedule needs the l1-Jacobi smoother (spectrum in (0, 1])");
+  c->gen += 1;
+  c->cheb_lo = lo;
+  set_sweep_weights(c);
   return 0;
 }
 
@@ -1349,6 +1389,8 @@ int maple_set_smoother(maple_ctx* c, int kind) {
   c->gen += 1;   // captured CG iterations of the old setting no longer apply
   if (kind != MAPLE_SMOOTH_JACOBI && kind != MAPLE_SMOOTH_L1_JACOBI) return fail(MAPLE_EINVAL, "unknown smoother");
   c->smoother = kind;
+  if (kind != MAPLE_SMOOTH_L1_JACOBI) c->cheb_lo = 0.0;   // its spectrum bound is the l1 one
+  set_sweep_weights(c);
   return 0;
 }
 

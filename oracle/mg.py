@@ -98,16 +98,30 @@ def translation_basis(nnodes: int, nd: int) -> np.ndarray:
     return Z
 
 
-class Hierarchy:
-    """Multigrid hierarchy for one density field and one physics (nd = 3 elastic, 1 thermal)."""
+def chebyshev_weights(k, lo, rising=True):
+    """Sweep weights of the Chebyshev smoother on [lo, 1]: the reciprocals of the k
+    Chebyshev nodes of that interval, so that prod_i (1 - w_i t) is the degree-k
+    Chebyshev polynomial of [lo, 1] normalised to 1 at t = 0 (rising or falling order)."""
+    i = np.arange(k)
+    nodes = 0.5 * (1.0 + lo) + 0.5 * (1.0 - lo) * np.cos((2.0 * i + 1.0) * np.pi / (2.0 * k))
+    w = np.sort(1.0 / nodes)
+    return w if rising else w[::-1]
 
-    def __init__(self, rho, Ke, dims, nlevels, omega=0.6, nsmooth=1):
+
+class Hierarchy:
+    """Multigrid hierarchy for one density field and one physics (nd = 3 elastic, 1 thermal).
+
+    weights: optional (pre, post) sequences of per-sweep smoother weights (the
+    Chebyshev schedule); by default every sweep takes omega."""
+
+    def __init__(self, rho, Ke, dims, nlevels, omega=0.6, nsmooth=1, weights=None):
         self.nd = Ke.shape[0] // 8
         nd = self.nd
         self.rho = np.asarray(rho, dtype=np.float64).ravel()
         self.Ke = Ke
         self.omega = omega
         self.nsmooth = nsmooth
+        self.wpre, self.wpost = weights if weights is not None else ([omega] * nsmooth, [omega] * nsmooth)
         self.dims = [tuple(dims)]
         for _ in range(nlevels - 1):
             self.dims.append(coarsen_dims(self.dims[-1]))
@@ -144,15 +158,15 @@ class Hierarchy:
         if l == len(self.dims) - 1:
             bb = b - self.Zc @ (self.Zc.T @ b)
             return sla.cho_solve(self.coarse_chol, bb)
-        dinv = self.omega / self.diag[l]
-        x = dinv * b
-        for _ in range(self.nsmooth - 1):
-            x = x + dinv * (b - self.apply(l, x))
+        dinv = 1.0 / self.diag[l]
+        x = self.wpre[0] * dinv * b
+        for w in self.wpre[1:]:
+            x = x + w * dinv * (b - self.apply(l, x))
         r = b - self.apply(l, x)
         e = self.vcycle(l + 1, self.P[l].T @ r)
         x = x + self.P[l] @ e
-        for _ in range(self.nsmooth):
-            x = x + dinv * (b - self.apply(l, x))
+        for w in self.wpost:
+            x = x + w * dinv * (b - self.apply(l, x))
         return x
 
 

@@ -201,6 +201,62 @@ def test_vcycle_matches_oracle(physics, smoother):
 
 
 @pytest.mark.parametrize("physics", ["elastic", "thermal"])
+@pytest.mark.parametrize("sweeps,lo", [(2, 0.1), (3, 0.05)])
+def test_chebyshev_vcycle_matches_oracle(physics, sweeps, lo):
+    """V-cycle with the Chebyshev sweep weights (l1-Jacobi spectrum, [lo, 1]) against the
+    oracle's V-cycle with the weights of oracle.mg.chebyshev_weights."""
+    print(f"[vcycle chebyshev] {physics} sweeps={sweeps} lo={lo}", flush=True)
+    r, nlev = 16, 3
+    rho = _rho(r, 13, 1e-3)
+    Ke = _Ke(physics, r)
+    nd = Ke.shape[0] // 8
+    nc = 6 if nd == 3 else 3
+    H = omg.Hierarchy(rho, Ke, (r, r, r), nlev, nsmooth=sweeps,
+                      weights=(omg.chebyshev_weights(sweeps, lo), omg.chebyshev_weights(sweeps, lo, rising=False)))
+    rows = np.tile(np.abs(Ke).sum(axis=1)[None, :], (r ** 3, 1)) * rho[:, None]
+    H.diag[0] = ogrid.scatter_elements(rows, (r, r, r), nd).ravel()
+    for l in range(1, nlev - 1):
+        H.diag[l] = np.asarray(abs(H.A[l]).sum(axis=1)).ravel()
+    cb = _batch(r, nlev, smoother="l1-jacobi")
+    cb.set_sweeps(sweeps, sweeps)
+    cb.set_chebyshev(lo)
+    rt = cb.mg_setup(physics, _t(rho))
+    rng = np.random.default_rng(3)
+    rv = rng.standard_normal((nc, nd, r ** 3))
+    rv -= rv.mean(axis=2, keepdims=True)
+    z = cb.mg_vcycle(physics, _t(rv)).cpu().numpy()
+    rd = _planes_to_dofs(rv)
+    want = _dofs_to_planes(np.stack([H.vcycle(0, rd[:, j]) for j in range(nc)], axis=1), nd)
+    assert np.linalg.norm(z - want) <= 1e-11 * np.linalg.norm(want)
+    del rt
+    cb.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_chebyshev_solve_matches_oracle(precision):
+    """Converged homogenization with the Chebyshev schedule (3 sweeps a side, [0.1, 1]): C_H
+    and kappa_H at the FP64 bar against the oracle."""
+    import maple_amd
+    from oracle import homog as ohomog
+    r = 16
+    print(f"[chebyshev solve] {precision}", flush=True)
+    rho = _rho(r, 17, 1e-9)
+    cb = maple_amd.CellBatch([r], nlevels=3, device=DEV, precision=precision)
+    cb.set_sweeps(3, 3)
+    cb.set_chebyshev(0.1)
+    CH, kap, _ = cb.homogenize(_t(rho), rtol=1e-10)
+    cs = ohomog.CellSolver((r, r, r), nlevels=3, rtol=1e-12)
+    CHo, _, _ = cs.elastic(rho)
+    ko, _, _ = cs.thermal(rho)
+    assert np.linalg.norm(CH[0].cpu().numpy() - CHo) <= 1e-9 * np.linalg.norm(CHo)
+    assert np.linalg.norm(kap[0].cpu().numpy() - ko) <= 1e-9 * np.linalg.norm(ko)
+    with pytest.raises(Exception):
+        cb.set_smoother("jacobi")
+        cb.set_chebyshev(0.1)   # the schedule needs the l1 bound of the spectrum
+    cb.close()
+
+
+@pytest.mark.parametrize("physics", ["elastic", "thermal"])
 def test_vcycle_fp32_follows_fp64(physics):
     print(f"[vcycle fp32] {physics}", flush=True)
     r, nlev = 16, 3

@@ -68,3 +68,20 @@ def test_pcg_equals_direct(physics):
     CHd = homog.homogenize(rho, Sd)
     assert np.linalg.norm(CH - CHd) < 1e-10 * np.linalg.norm(CHd)
     assert max(cs.iters[physics]) < 200
+
+
+@pytest.mark.parametrize("k,lo", [(1, 0.1), (2, 0.1), (3, 0.1), (3, 0.05), (4, 0.2)])
+def test_chebyshev_weights_give_the_chebyshev_polynomial(k, lo):
+    """prod_i (1 - w_i t) is T_k((1 + lo - 2t) / (1 - lo)) / T_k((1 + lo) / (1 - lo)) (numpy's
+    Chebyshev series), so its largest magnitude on [lo, 1] is 1 / T_k((1 + lo) / (1 - lo))."""
+    from numpy.polynomial import chebyshev as C
+    w = mg.chebyshev_weights(k, lo)
+    assert np.all(np.diff(w) > 0) and np.all(mg.chebyshev_weights(k, lo, rising=False) == w[::-1])
+    t = np.linspace(0.0, 1.0, 401)
+    p = np.prod([1.0 - wi * t for wi in w], axis=0)
+    Tk = np.zeros(k + 1)
+    Tk[k] = 1.0
+    want = C.chebval((1.0 + lo - 2.0 * t) / (1.0 - lo), Tk) / C.chebval((1.0 + lo) / (1.0 - lo), Tk)
+    assert np.abs(p - want).max() < 1e-12
+    inside = (t >= lo)
+    assert np.abs(p[inside]).max() <= 1.0 / C.chebval((1.0 + lo) / (1.0 - lo), Tk) + 1e-12
