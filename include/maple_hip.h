@@ -169,6 +169,18 @@ int maple_set_mixed_fallback(maple_ctx* ctx, int iters);
  * which rebuilds its whole hierarchy on every solve, EBE_GMG.py:790-849). */
 int maple_set_coarse_refresh(maple_ctx* ctx, int every);
 
+/* Start each solve projected on the last m solutions (0 = off, the default; at most
+ * 5).  Per (cell, case) the solve starts from the combination of those solutions
+ * and a supplied start (maple_set_start) that minimises the energy norm of the
+ * error, (u* - u0)^T K (u* - u0): c solves the m x m (or m+1) Galerkin system
+ * (h_a^T K h_b) c = h_a^T f, leaving out vectors that are numerically combinations
+ * of newer ones, so the previous fields (warm start) and the supplied start are
+ * special cases.  Each solve costs one operator application per basis vector; the
+ * context holds 2m+1 copies of each physics' fields.  Solves by the per-case CG
+ * only (not block CG).  Changing m clears the history.  No counterpart in the
+ * reference, which starts every solve from zero (EBE_GMG.py:852-858). */
+int maple_set_start_history(maple_ctx* ctx, int m);
+
 /* Kernel variants for the tuner (maple_amd/tune.py times them and keeps the
  * fastest): MAPLE_KERNEL_FINE_OPERATOR (the level-0 elastic operator of
  * EBEOperator.matvec, EBE_GMG.py:412-430), variant 0 (element rows per pencil
@@ -322,7 +334,8 @@ int maple_set_fields(maple_ctx* ctx, int physics, const double* u_dev, void* str
 int maple_set_start(maple_ctx* ctx, int physics, const double* u_dev, void* stream);
 
 /* Start chosen by the last solve of one physics, per (cell, case): 0 zero,
- * 1 previous fields, 2 supplied start (ncells*6 or ncells*3 ints). */
+ * 1 previous fields, 2 supplied start, 3 projected start (maple_set_start_history)
+ * (ncells*6 or ncells*3 ints). */
 int maple_start_sources(maple_ctx* ctx, int physics, int* src_host);
 
 /* fine_dev = P coarse_dev: periodic trilinear prolongation of plane-major

@@ -56,6 +56,7 @@ SIGNATURES = {
     "maple_set_chebyshev": (_I, [_P, C.c_double]),
     "maple_graph_counts": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "maple_set_coarse_refresh": (_I, [_P, _I]),
+    "maple_set_start_history": (_I, [_P, _I]),
     "maple_set_variant": (_I, [_P, _I, _I]),
     "maple_homogenize": (_I, [_P, _P, _I, _D, _I, _P, _P, _IP, _P]),
     "maple_sensitivity": (_I, [_P, _P, _P, _P, _P]),

@@ -167,6 +167,11 @@ class CellBatch:
         """Rebuild the coarsest dense inverse every `every` homogenizations (1 = always)."""
         L.check(self.lib.maple_set_coarse_refresh(self._h, int(every)))
 
+    def set_start_history(self, m: int = 0):
+        """Start each solve from the energy-optimal combination of the last `m` solutions
+        and a supplied start (0: off); clears the history."""
+        L.check(self.lib.maple_set_start_history(self._h, int(m)))
+
     def set_element_matrices(self, cell: int, Ke24=None, Fe24=None, X0_24=None, Ke8=None, Fe8=None, X0_8=None):
         def p(a):
             if a is None:
@@ -278,7 +283,8 @@ class CellBatch:
         L.check(self.lib.maple_set_start(self._h, PHYS[physics], C.c_void_p(u.data_ptr()), self._stream()))
 
     def start_sources(self, physics: str) -> np.ndarray:
-        """Start chosen by the last solve, (ncells, ncase): 0 zero, 1 previous fields, 2 supplied."""
+        """Start chosen by the last solve, (ncells, ncase): 0 zero, 1 previous fields, 2 supplied,
+        3 projected on earlier solutions."""
         out = (C.c_int * (self.ncells * NCASE[physics]))()
         L.check(self.lib.maple_start_sources(self._h, PHYS[physics], out))
         return np.array(out, dtype=np.int32).reshape(self.ncells, NCASE[physics])

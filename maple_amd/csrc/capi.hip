@@ -127,6 +127,14 @@ struct Phys {
   size_t stage_bytes = 0;
   int stage_slots = 0, stage_next = 0, nlive = 0;
   bool have_start = false;
+  // start projected on earlier solutions (maple_set_start_history): a ring of the last
+  // hist_m solutions, the products K h of a solve's basis, the Gram partials and the
+  // coefficients
+  double* hist = nullptr;
+  double* khist = nullptr;
+  double* gpart = nullptr;
+  double* pcoef = nullptr;
+  int nhist = 0, hpos = 0;
   double* isl_inv = nullptr;  // island correction: inverse blocks of this physics, per island
   int* isl_valid = nullptr;
   long long setups = 0;    // hierarchy builds so far (drives the coarsest-inverse refresh)
@@ -166,6 +174,8 @@ struct maple_ctx {
   int block_max = 200; // block CG iterations before per-case CG takes over
   int coarse_refresh = 1;   // rebuild the coarsest inverse every this many homogenizations
   int fallback = 300;       // mixed form: FP64 V-cycle after this many CG iterations (0: never)
+  int hist_m = 0;           // > 0: solves start projected on the last hist_m solutions
+  std::vector<void*> hist_allocs;
   OCState* ocst = nullptr;
   OCState* ocst_h = nullptr;
   double* oc_a = nullptr;
@@ -791,6 +801,38 @@ int run_segment(maple_ctx* c, Phys& P, SegGraph& G, unsigned long long key, hipS
   return 0;
 }
 
+// Start projected on earlier solutions: the fields of the previous solve join the ring of
+// the last hist_m solutions; the basis is the ring, newest first, and a supplied start.
+// Per (cell, case) the start is the combination of the basis that minimises the energy
+// norm of the error (proj_start); it and its residual go to P.ustart and P.q with
+// (r, r) in CaseScalars::rrs, ready for pcg_pick_start.  P.f must hold the right-hand side.
+int projected_start(maple_ctx* c, Phys& P, const double* rho, hipStream_t s, bool* done) {
+  *done = false;
+  const Level& L0 = c->lv[0];
+  const size_t n0 = (size_t)P.nc * P.nd * L0.Ntot;
+  const int m = c->hist_m;
+  if (P.have_fields) {
+    HIPCHK(hipMemcpyAsync(P.hist + (size_t)P.hpos * n0, P.u, n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    P.hpos = (P.hpos + 1) % m;
+    P.nhist = std::min(P.nhist + 1, m);
+  }
+  ProjBasis B{};
+  int k = 0;
+  for (int t = 0; t < P.nhist; ++t) B.h[k++] = P.hist + (size_t)((P.hpos - 1 - t + 2 * m) % m) * n0;
+  if (P.have_start) B.h[k++] = P.ustart;
+  if (k == 0) return 0;
+  for (int a = 0; a < k; ++a) {
+    double* g = P.khist + (size_t)a * n0;
+    apply_fine64(c, P, 0, rho, B.h[a], nullptr, g, nullptr, s);
+    B.g[a] = g;
+  }
+  if (proj_start(L0.node, P.nd, P.nc, k, B, P.f, P.gpart, P.pcoef, P.ustart, P.q, P.partial, s))
+    return fail(MAPLE_EINVAL, "projected start: unsupported basis or field shape");
+  reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RRS, s);
+  *done = true;
+  return 0;
+}
+
 int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStream_t s, bool force_prev = false) {
   Level& L0 = c->lv[0];
   const bool flex = c->flexible < 0 ? c->prec != MAPLE_PREC_FP64 : c->flexible != 0;
@@ -806,19 +848,25 @@ int pcg(maple_ctx* c, Phys& P, const double* rho, double rtol, int maxit, hipStr
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.f, P.f, P.partial, s);
   reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_FF, s);
   // start: per (cell, case) the smallest residual among zero, the previous fields
-  // (warm start) and a supplied start (learned warm start)
-  const bool have_prev = (c->warm || force_prev) && P.have_fields;
+  // (warm start) and a supplied start (learned warm start); with a solution history, the
+  // previous fields and the supplied start are basis vectors of the projected start
+  // instead.  A restart from the fields of an unfinished solve (force_prev) is no new
+  // solution and keeps them.
+  bool projected = false;
+  if (c->hist_m > 0 && !force_prev) TRY(projected_start(c, P, rho, s, &projected));
+  const bool have_prev = !projected && (c->warm || force_prev) && P.have_fields;
   if (have_prev) {
     apply_fine64(c, P, 2, rho, P.u, P.f, P.r, nullptr, s);
     vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, P.partial, s);
     reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RR, s);
   }
-  if (P.have_start) {
+  if (P.have_start && !projected) {
     apply_fine64(c, P, 2, rho, P.ustart, P.f, P.q, nullptr, s);
     vec_dot<double, double>(L0.node, P.nd, P.nc, P.q, P.q, P.partial, s);
     reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RRS, s);
   }
-  pcg_pick_start(L0.node, c->ncells, P.nd, P.nc, have_prev, P.have_start, P.sc, P.u, P.r, P.ustart, P.q, P.f, s);
+  pcg_pick_start(L0.node, c->ncells, P.nd, P.nc, have_prev, P.have_start || projected, P.sc, P.u, P.r, P.ustart,
+                 P.q, P.f, s, projected ? 3 : 2);
   P.have_start = false;
   vec_dot<double, double>(L0.node, P.nd, P.nc, P.r, P.r, P.partial, s);
   reduce_partials(L0.node, P.nc, P.partial, P.sc, SC_RR, s);
@@ -1206,6 +1254,7 @@ int maple_ctx_destroy(maple_ctx* c) {
   (void)hipDeviceSynchronize();
   for (void* p : c->allocs) (void)hipFree(p);
   for (void* p : c->cav_allocs) (void)hipFree(p);
+  for (void* p : c->hist_allocs) (void)hipFree(p);
   for (void* p : c->host_allocs) (void)hipHostFree(p);
   for (Phys& P : c->ph) {
     for (int k = 0; k < 2; ++k)
@@ -1284,6 +1333,37 @@ int maple_set_coarse_refresh(maple_ctx* c, int every) {
   if (every < 1) return fail(MAPLE_EINVAL, "refresh interval must be >= 1");
   c->coarse_refresh = every;
   for (Phys& P : c->ph) P.setups = 0;
+  return 0;
+}
+
+int maple_set_start_history(maple_ctx* c, int m) {
+  TRY(check_ctx(c));
+  if (m < 0 || m > MAPLE_PROJ_MAX - 1) return fail(MAPLE_EINVAL, "history length must lie in [0, 5]");
+  TRY(set_device(c));
+  HIPCHK(hipDeviceSynchronize());
+  for (void* p : c->hist_allocs) (void)hipFree(p);
+  c->hist_allocs.clear();
+  c->hist_m = 0;
+  for (Phys& P : c->ph) {
+    P.hist = P.khist = P.gpart = P.pcoef = nullptr;
+    P.nhist = P.hpos = 0;
+  }
+  if (m == 0) return 0;
+  auto grab = [&](double** p, size_t n) -> int {
+    TRY(dalloc(c, p, n));
+    c->allocs.pop_back();          // owned by the history list, freed when it is resized
+    c->hist_allocs.push_back((void*)*p);
+    return 0;
+  };
+  const int maxb = c->lv[0].node.nblocks;
+  for (Phys& P : c->ph) {
+    const size_t n0 = (size_t)P.nc * P.nd * c->lv[0].Ntot;
+    TRY(grab(&P.hist, (size_t)m * n0));
+    TRY(grab(&P.khist, (size_t)(m + 1) * n0));
+    TRY(grab(&P.gpart, (size_t)maxb * P.nc * MAPLE_PROJ_NG));
+    TRY(grab(&P.pcoef, (size_t)c->ncells * P.nc * MAPLE_PROJ_MAX));
+  }
+  c->hist_m = m;
   return 0;
 }
 

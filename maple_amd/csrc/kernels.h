@@ -137,7 +137,8 @@ struct CaseScalars {   // one per (cell, case)
   double rrs;          // squared residual of a supplied start (learned warm start)
   double alpha, beta;
   int done, iters;
-  int src;             // start of the last solve: 0 zero, 1 previous fields, 2 supplied start
+  int src;             // start of the last solve: 0 zero, 1 previous fields, 2 supplied start,
+                       // 3 projected on earlier solutions
   int brk;             // stopped early: 1 breakdown (p^T K p <= 0 on MAPLE_CG_RESTARTS directions in
                        // a row), 2 stagnation (no new residual minimum in MAPLE_CG_STALL iterations,
                        // or the residual 1e6 times above its minimum)
@@ -191,11 +192,28 @@ void vec_dot(const Launch& L, int nd, int ncase, const Ta* a, const Tb* b, doubl
 // u += alpha p; r -= alpha q; partial (r, r)
 void pcg_update_ur(const Launch& L, int nd, int ncase, const CaseScalars* sc, const double* p, const double* q,
                    double* u, double* r, double* partial, hipStream_t s);
+// Start projected on earlier solutions: basis vectors h_a and their products g_a = K h_a
+// (device pointers, a < k <= MAPLE_PROJ_MAX).  Per (cell, case) the combination
+// sum_a c_a h_a that minimises the energy norm of the error (G c = b with G_ab = (h_a, K h_b),
+// b_a = (h_a, f); vectors whose Cholesky pivot falls below MAPLE_PROJ_TOL G_aa are left out)
+// goes to us, its residual f - sum_a c_a g_a to rs, (rs, rs) per case to partial.
+// gpart: L.nblocks x ncase x MAPLE_PROJ_NG doubles; coef: ncells x ncase x MAPLE_PROJ_MAX.
+#define MAPLE_PROJ_MAX 6
+#define MAPLE_PROJ_NG (MAPLE_PROJ_MAX * (MAPLE_PROJ_MAX + 1) / 2 + MAPLE_PROJ_MAX)
+#define MAPLE_PROJ_TOL 1e-10
+struct ProjBasis {
+  const double* h[MAPLE_PROJ_MAX];
+  const double* g[MAPLE_PROJ_MAX];
+};
+int proj_start(const Launch& L, int nd, int ncase, int k, const ProjBasis& B, const double* f, double* gpart,
+               double* coef, double* us, double* rs, double* partial, hipStream_t s);
 // Start of a solve, per (cell, case): the candidate with the smallest residual
 // among zero (residual f), the previous fields u (residual r, if have_prev) and a
 // supplied start us (residual rs, if have_start); u and r receive the choice.
+// start_src: the source recorded for the supplied start (2 supplied, 3 projected).
 void pcg_pick_start(const Launch& L, int ncells, int nd, int ncase, bool have_prev, bool have_start, CaseScalars* sc,
-                    double* u, double* r, const double* us, const double* rs, const double* f, hipStream_t s);
+                    double* u, double* r, const double* us, const double* rs, const double* f, hipStream_t s,
+                    int start_src = 2);
 // (r_j, z_j) into partial[blk*6 + j], and the sums of z over the block's nodes per (case,
 // component) into zpart[blk*18 + j*nd + i]
 template <typename Tz>

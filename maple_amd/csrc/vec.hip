@@ -370,7 +370,7 @@ int pcg_update_p32(const Launch& L, int nd, int ncase, const CaseScalars* sc, co
   return 0;
 }
 
-__global__ void k_pick_src(int n, int have_prev, int have_start, CaseScalars* __restrict__ sc) {
+__global__ void k_pick_src(int n, int have_prev, int have_start, int start_src, CaseScalars* __restrict__ sc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   CaseScalars& c = sc[t];
@@ -380,7 +380,7 @@ __global__ void k_pick_src(int n, int have_prev, int have_start, CaseScalars* __
     src = 1;
     best = c.rr;
   }
-  if (have_start && c.rrs < best) src = 2;
+  if (have_start && c.rrs < best) src = start_src;
   c.src = src;
 }
 
@@ -395,16 +395,17 @@ __global__ __launch_bounds__(256) void k_apply_start(Launch L, int nd, int ncase
     if (src == 1) continue;
     for (int i = 0; i < nd; ++i) {
       const long long idx = (long long)(j * nd + i) * L.Ntot + gid;
-      u[idx] = src == 2 ? us[idx] : 0.0;
-      r[idx] = src == 2 ? rs[idx] : f[idx];
+      u[idx] = src >= 2 ? us[idx] : 0.0;
+      r[idx] = src >= 2 ? rs[idx] : f[idx];
     }
   }
 }
 
 void pcg_pick_start(const Launch& L, int ncells, int nd, int ncase, bool have_prev, bool have_start, CaseScalars* sc,
-                    double* u, double* r, const double* us, const double* rs, const double* f, hipStream_t s) {
+                    double* u, double* r, const double* us, const double* rs, const double* f, hipStream_t s,
+                    int start_src) {
   const int n = ncells * ncase;
-  k_pick_src<<<(n + 63) / 64, 64, 0, s>>>(n, have_prev ? 1 : 0, have_start ? 1 : 0, sc);
+  k_pick_src<<<(n + 63) / 64, 64, 0, s>>>(n, have_prev ? 1 : 0, have_start ? 1 : 0, start_src, sc);
   k_apply_start<<<L.nblocks, 256, 0, s>>>(L, nd, ncase, sc, u, r, us, rs, f);
 }
 
@@ -573,3 +574,149 @@ void vec_zero(long long n, T* dst, hipStream_t s) {
 template void vec_zero<double>(long long, double*, hipStream_t);
 template void vec_zero<float>(long long, float*, hipStream_t);
 template void vec_zero<int>(long long, int*, hipStream_t);
+
+// ---- start projected on earlier solutions ------------------------------------
+// Per (cell, case) j, with basis vectors h_a (a < k) and their products g_a = K h_a:
+// the packed upper triangle of G_ab = (h_a, g_b) (a <= b, rows in order) and
+// b_a = (h_a, f), MAPLE_PROJ_NG values per block and case.  Slots a >= k read nothing
+// and sum zero.  The cases run one after the other so a thread holds one case's sums.
+template <int ND>
+__global__ __launch_bounds__(256) void k_proj_gram(Launch L, int ncase, int k, ProjBasis B,
+                                                   const double* __restrict__ f, double* __restrict__ partial) {
+  int ci; long long gid = 0;
+  const bool ok = vnode(L, ci, gid);
+  for (int j = 0; j < ncase; ++j) {
+    double v[MAPLE_PROJ_NG];
+#pragma unroll
+    for (int t = 0; t < MAPLE_PROJ_NG; ++t) v[t] = 0.0;
+    if (ok) {
+#pragma unroll
+      for (int i = 0; i < ND; ++i) {
+        const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
+        double hv[MAPLE_PROJ_MAX], gv[MAPLE_PROJ_MAX];
+#pragma unroll
+        for (int a = 0; a < MAPLE_PROJ_MAX; ++a) {
+          hv[a] = a < k ? B.h[a][idx] : 0.0;
+          gv[a] = a < k ? B.g[a][idx] : 0.0;
+        }
+        const double fv = f[idx];
+        int t = 0;
+#pragma unroll
+        for (int a = 0; a < MAPLE_PROJ_MAX; ++a)
+#pragma unroll
+          for (int b = a; b < MAPLE_PROJ_MAX; ++b) v[t++] += hv[a] * gv[b];
+#pragma unroll
+        for (int a = 0; a < MAPLE_PROJ_MAX; ++a) v[t++] += hv[a] * fv;
+      }
+    }
+    block_sum<MAPLE_PROJ_NG>(v, partial + ((long long)lbid() * ncase + j) * MAPLE_PROJ_NG);
+  }
+}
+
+__device__ __forceinline__ int proj_packed(int a, int b) {   // a <= b
+  return a * MAPLE_PROJ_MAX - a * (a - 1) / 2 + (b - a);
+}
+
+// One wavefront per (cell, case): sum the cell's block partials, then (lane 0) the
+// coefficients c minimising the energy norm of the error of sum_a c_a h_a, i.e. G c = b
+// on the basis vectors that are not (numerically) combinations of earlier ones: a
+// Cholesky factorisation in basis order that drops a vector whose new pivot is below
+// tol times its own G_aa (a zero or nearly dependent vector gets c_a = 0).
+__global__ __launch_bounds__(64) void k_proj_coef(const LevelCell* __restrict__ cells,
+                                                  const int* __restrict__ blk_start, int ncase, int k,
+                                                  const double* __restrict__ partial, double tol,
+                                                  double* __restrict__ coef) {
+  const int kk = blockIdx.x, j = blockIdx.y, ci = cells[kk].id;
+  const int b0 = blk_start[kk], b1 = blk_start[kk + 1];
+  double s[MAPLE_PROJ_NG];
+#pragma unroll
+  for (int t = 0; t < MAPLE_PROJ_NG; ++t) {
+    double v = 0.0;
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += 64) v += partial[((long long)b * ncase + j) * MAPLE_PROJ_NG + t];
+    s[t] = wave_sum(v);
+  }
+  if (threadIdx.x != 0) return;
+  const double* rhs = s + MAPLE_PROJ_MAX * (MAPLE_PROJ_MAX + 1) / 2;
+  double Lm[MAPLE_PROJ_MAX][MAPLE_PROJ_MAX];
+  int keep[MAPLE_PROJ_MAX];
+  int nk = 0;
+  for (int a = 0; a < k; ++a) {
+    const double gaa = s[proj_packed(a, a)];
+    if (!(gaa > 0.0)) continue;
+    double l[MAPLE_PROJ_MAX];
+    double d = gaa;
+    for (int t = 0; t < nk; ++t) {
+      const int b = keep[t];
+      double v = s[b < a ? proj_packed(b, a) : proj_packed(a, b)];
+      for (int q = 0; q < t; ++q) v -= l[q] * Lm[t][q];
+      l[t] = v / Lm[t][t];
+      d -= l[t] * l[t];
+    }
+    if (!(d > tol * gaa)) continue;
+    for (int t = 0; t < nk; ++t) Lm[nk][t] = l[t];
+    Lm[nk][nk] = sqrt(d);
+    keep[nk++] = a;
+  }
+  double y[MAPLE_PROJ_MAX], x[MAPLE_PROJ_MAX];
+  for (int t = 0; t < nk; ++t) {
+    double v = rhs[keep[t]];
+    for (int q = 0; q < t; ++q) v -= Lm[t][q] * y[q];
+    y[t] = v / Lm[t][t];
+  }
+  for (int t = nk - 1; t >= 0; --t) {
+    double v = y[t];
+    for (int q = t + 1; q < nk; ++q) v -= Lm[q][t] * x[q];
+    x[t] = v / Lm[t][t];
+  }
+  double* out = coef + ((long long)ci * ncase + j) * MAPLE_PROJ_MAX;
+  for (int a = 0; a < MAPLE_PROJ_MAX; ++a) out[a] = 0.0;
+  for (int t = 0; t < nk; ++t) out[keep[t]] = x[t];
+}
+
+// us = sum_a c_a h_a, rs = f - sum_a c_a g_a, partial (rs, rs) per case.  us may be one
+// of the basis vectors (each thread reads its node's values before writing them).
+template <int ND>
+__global__ __launch_bounds__(256) void k_proj_combine(Launch L, int ncase, int k, ProjBasis B,
+                                                      const double* __restrict__ coef, const double* __restrict__ f,
+                                                      double* us, double* __restrict__ rs,
+                                                      double* __restrict__ partial) {
+  int ci; long long gid = 0;
+  const bool ok = vnode(L, ci, gid);
+  double v[MAPLE_MAX_CASES];
+#pragma unroll
+  for (int j = 0; j < MAPLE_MAX_CASES; ++j) v[j] = 0.0;
+  if (ok) {
+    for (int j = 0; j < ncase; ++j) {
+      const double* cj = coef + ((long long)ci * ncase + j) * MAPLE_PROJ_MAX;
+      double c[MAPLE_PROJ_MAX];
+#pragma unroll
+      for (int a = 0; a < MAPLE_PROJ_MAX; ++a) c[a] = a < k ? cj[a] : 0.0;
+#pragma unroll
+      for (int i = 0; i < ND; ++i) {
+        const long long idx = (long long)(j * ND + i) * L.Ntot + gid;
+        double u = 0.0, r = f[idx];
+#pragma unroll
+        for (int a = 0; a < MAPLE_PROJ_MAX; ++a)
+          if (a < k) {
+            u += c[a] * B.h[a][idx];
+            r -= c[a] * B.g[a][idx];
+          }
+        us[idx] = u;
+        rs[idx] = r;
+        v[j] += r * r;
+      }
+    }
+  }
+  block_sum<MAPLE_MAX_CASES>(v, partial + (long long)lbid() * MAPLE_MAX_CASES);
+}
+
+int proj_start(const Launch& L, int nd, int ncase, int k, const ProjBasis& B, const double* f, double* gpart,
+               double* coef, double* us, double* rs, double* partial, hipStream_t s) {
+  if (k < 1 || k > MAPLE_PROJ_MAX || ncase > MAPLE_MAX_CASES || (nd != 1 && nd != 3)) return -1;
+  if (nd == 3) k_proj_gram<3><<<L.nblocks, 256, 0, s>>>(L, ncase, k, B, f, gpart);
+  else k_proj_gram<1><<<L.nblocks, 256, 0, s>>>(L, ncase, k, B, f, gpart);
+  k_proj_coef<<<dim3(L.ncells, ncase), 64, 0, s>>>(L.cells, L.blk_start, ncase, k, gpart, MAPLE_PROJ_TOL, coef);
+  if (nd == 3) k_proj_combine<3><<<L.nblocks, 256, 0, s>>>(L, ncase, k, B, coef, f, us, rs, partial);
+  else k_proj_combine<1><<<L.nblocks, 256, 0, s>>>(L, ncase, k, B, coef, f, us, rs, partial);
+  return 0;
+}
