@@ -21,9 +21,11 @@ Forms (one JSON line; ``value`` is the headline form):
                island correction (small solid islands solved exactly beside the
                V-cycle) and its coarsest dense inverse rebuilt every third
                density (an earlier density's SPD inverse in between), each
-               solve starting from the best of zero, the previous
-               iterate's fields and those fields plus the change the learned
-               network predicts between the two densities.
+               solve starting from the combination of the last 4 solutions and
+               the learned candidate (the previous fields plus the change the
+               network predicts between the two densities) that minimises the
+               energy norm of the error (maple_set_start_history), or from zero
+               where that has the smaller residual.
                value = cells x iterations / s.
                The V-cycle smooths with 3+3 l1-Jacobi sweeps whose weights are the
                reciprocal Chebyshev nodes of [0.1, 1] (maple_set_chebyshev).
@@ -35,7 +37,9 @@ Forms (one JSON line; ``value`` is the headline form):
 * optional (--forms): single-cell and batched ablations (``fp64-warm``,
   ``fp64-islands``, ``fp64-block``, ``mixed-warm``, ``mixed-warm-block``,
   ``mixed-warm-learned``, ``batched-mixed-warm``, ``batched-mixed-warm-islands``,
-  ``batched-mixed-warm-block``, ``headline-block``).
+  ``batched-mixed-warm-block``, ``headline-block``, ``headline-no-history``: the
+  headline with the best of zero / previous fields / learned candidate instead of the
+  projection, ``headline-no-learned``: the projection without the learned candidate).
 
 Beside the headline (each can be switched off):
 * ``config4``: the 64-cell mixed-resolution batch (16 x 32^3, 32 x 48^3, 16 x 64^3,
@@ -102,7 +106,7 @@ def _x0(ncells):
 def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", warm: bool = False,
              learned: bool = False, block: bool = False, islands: bool = False, groups: int = 1, refresh: int = 1,
              maxit: int = 5000, profile_kernel: bool = True, tune: bool = True, cheb: float = 0.0, sweeps: int = 2,
-             graphs: bool = False):
+             graphs: bool = False, hist: int = 0):
     """groups > 1: the cells advance in that many concurrent groups (maple_amd.concurrent),
     each its own batch, stream and host thread; every cell's trajectory is unchanged."""
     import numpy as np
@@ -127,9 +131,12 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
                 cells.set_chebyshev(cheb)
             if graphs:
                 cells.set_graphs(True)
+            if hist:
+                cells.set_start_history(hist)
             if tune:
                 from maple_amd import tune as mt
-                tuned.update({k: v for k, (v, _) in mt.autotune(cells).items()})
+                # groups share the device: time the candidates as that many concurrent copies
+                tuned.update({k: v for k, (v, _) in mt.autotune(cells, streams=groups).items()})
             if groups > 1:
                 # the groups already overlap on the device: each solves its two physics in turn
                 # on its own stream (4 hardware queues per process)
@@ -239,6 +246,7 @@ def run_config4(steps: int, precision: str = "mixed", maxit: int = 3000):
     cells.set_coarse_refresh(3)
     cells.set_sweeps(3, 3)
     cells.set_chebyshev(0.1)
+    cells.set_start_history(4)
     from maple_amd import tune as mt
     tuned = {k: v for k, (v, _) in mt.autotune(cells).items()}
     drv = maple_amd.OCDriver(cells, [c["weights"] for c in spec], [c["volfrac"] for c in spec], move=0.2, xmin=1e-3,
@@ -393,9 +401,13 @@ def cpu_baseline(n_iter: int, warmup: int = 2):
 FORMS = {
     "plain": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False, islands=False),
     "headline": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True, groups=4,
-                     refresh=3, cheb=0.1, sweeps=3),
+                     refresh=3, cheb=0.1, sweeps=3, hist=4),
     "headline-lockstep": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True,
-                              refresh=3, cheb=0.1, sweeps=3),
+                              refresh=3, cheb=0.1, sweeps=3, hist=4),
+    "headline-no-history": dict(ncells=16, precision="mixed", warm=True, learned=True, block=False, islands=True,
+                                groups=4, refresh=3, cheb=0.1, sweeps=3),
+    "headline-no-learned": dict(ncells=16, precision="mixed", warm=True, learned=False, block=False, islands=True,
+                                groups=4, refresh=3, cheb=0.1, sweeps=3, hist=4),
     "plain-cheb": dict(ncells=1, precision="fp64", warm=False, learned=False, block=False, islands=False, cheb=0.1,
                        sweeps=3),
     "headline-block": dict(ncells=16, precision="mixed", warm=True, learned=True, block=True, islands=True,
@@ -468,6 +480,7 @@ def main():
                    "smoother": (f"l1-Jacobi, {spec.get('sweeps', 2)}+{spec.get('sweeps', 2)} sweeps, Chebyshev weights "
                                 f"over [{spec['cheb']}, 1]") if spec.get("cheb") else "l1-Jacobi, 2+2 sweeps, omega 1.6",
                    "coarsest_inverse_every": spec.get("refresh", 1),
+                   "start_history": spec.get("hist", 0),
                    "resolution": R, "oc_iterations_timed": TRAJ,
                    "levels": NLEVELS, "rtol": 1e-6},
         "ch_rel_err_vs_fp64_tight_solve": g["ch_rel_err"],

@@ -13,9 +13,17 @@ The choice depends on the cell mix (for a batch with cells whose nz is not 64 th
 lane-rotation variants do not apply and the 2x1 node blocks usually win), so it is
 kept per batch shape (cell sizes, precision, level count) for the process:
 ``autotune(cells)`` on a second batch of the same shape reuses it.
+
+``streams > 1`` times every variant as that many concurrent copies on separate
+streams, for batches that will share the device with as many others (concurrent
+groups): a kernel that wins alone can lose there.  The FP32 MFMA operator (variant
+9) is such a case: alone it is faster than the vector node-block form for 4-cell
+64^3 groups, beside three other groups it is slower (its LDS and two-wavefront
+occupancy leave less room for their kernels).
 """
 from __future__ import annotations
 
+import time
 from typing import Dict, Tuple
 
 import torch
@@ -32,27 +40,44 @@ KERNELS = {
 _CACHE: Dict[Tuple, Dict[str, int]] = {}
 
 
-def _time(fn, reps: int) -> float:
-    fn()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
+def _time(fn, reps: int, streams=None) -> float:
+    """ms per repetition; with streams, every repetition issues fn once on each stream."""
+    if not streams:
         fn()
-    e1.record()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def once():
+        for st in streams:
+            with torch.cuda.stream(st):
+                fn()
+    torch.cuda.synchronize()   # the inputs were made on the caller's stream
+    once()
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        once()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
 
 
-def shape_key(cells: CellBatch) -> Tuple:
-    return (tuple(cells.dims), cells.precision, cells.nlevels)
+def shape_key(cells: CellBatch, streams: int = 1) -> Tuple:
+    return (tuple(cells.dims), cells.precision, cells.nlevels, int(streams))
 
 
-def autotune(cells: CellBatch, reps: int = 5, seed: int = 0, force: bool = False) -> dict:
+def autotune(cells: CellBatch, reps: int = 5, seed: int = 0, force: bool = False, streams: int = 1) -> dict:
     """Pick the fastest variant of every tunable kernel for this batch shape and set it;
-    returns {kernel: (variant, {variant: ms})} ({kernel: (variant, None)} from the cache)."""
-    key = shape_key(cells)
+    returns {kernel: (variant, {variant: ms})} ({kernel: (variant, None)} from the cache).
+    streams > 1: time each variant as that many concurrent copies (one per stream)."""
+    key = shape_key(cells, streams)
+    sts = [torch.cuda.Stream(device=cells.device) for _ in range(streams)] if streams > 1 else None
     if not force and key in _CACHE:
         for name, v in _CACHE[key].items():
             cells.set_variant(KERNELS[name][0], v)
@@ -84,7 +109,7 @@ def autotune(cells: CellBatch, reps: int = 5, seed: int = 0, force: bool = False
         times = {}
         for v in variants:
             cells.set_variant(kid, v)
-            times[v] = _time(roles[name], reps)
+            times[v] = _time(roles[name], reps, sts)
         best = min(times, key=times.get)
         cells.set_variant(kid, best)
         out[name] = (best, times)
