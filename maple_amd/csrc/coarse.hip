@@ -442,7 +442,29 @@ __global__ __launch_bounds__(256) void k_prolong_add(const LevelCell* __restrict
 // One thread per fine node for every (case, component) plane: the coarse
 // neighbours and weights are found once; the z parity varies across a wavefront,
 // so both coarse z neighbours are read with weights (1, 0) or (1/2, 1/2) instead of
-// branching (x and y parities are uniform over a z-pencil).
+// branching.  The x and y parities are uniform over a z-pencil (a whole wavefront
+// when nz = 64), so the kernel branches once on them into a body whose number of
+// coarse columns (1, 2 or 4) is a compile-time constant: every load of a thread is
+// then independent of the others and in flight at once.
+template <int ND, int NC, int NK, typename T>
+__device__ __forceinline__ void prolong_columns(const long long (&cn)[4], int bz, int cz1, T wxy, T wz0, T wz1,
+                                                const T* __restrict__ ec, long long Ntot_c, T* __restrict__ uf,
+                                                long long Ntot_f, long long gid, const int* __restrict__ done, int ndc,
+                                                int ci) {
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    if (case_done(done, ndc, ci, j)) continue;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const T* __restrict__ e = ec + (long long)(j * ND + d) * Ntot_c;
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < NK; ++k) acc += wz0 * e[cn[k] + bz] + wz1 * e[cn[k] + cz1];
+      uf[(long long)(j * ND + d) * Ntot_f + gid] += wxy * acc;
+    }
+  }
+}
+
 template <int ND, int NC, typename T>
 __global__ __launch_bounds__(256) void k_prolong_add_t(const LevelCell* __restrict__ fcells,
                                                        const int* __restrict__ blk_start, int ncells, long long Ntot_f,
@@ -469,17 +491,9 @@ __global__ __launch_bounds__(256) void k_prolong_add_t(const LevelCell* __restri
       if ((ox && !rx) || (oy && !ry)) continue;
       cn[nk++] = cc.off + ((long long)cx[ox] * cc.ny + cy[oy]) * cc.nz;
     }
-#pragma unroll
-  for (int j = 0; j < NC; ++j) {
-    if (case_done(done, ndc, ci, j)) continue;
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      const T* __restrict__ e = ec + (long long)(j * ND + d) * Ntot_c;
-      T acc = T(0);
-      for (int k = 0; k < nk; ++k) acc += wz0 * e[cn[k] + bz] + wz1 * e[cn[k] + cz1];
-      uf[(long long)(j * ND + d) * Ntot_f + gid] += wxy * acc;
-    }
-  }
+  if (nk == 4) prolong_columns<ND, NC, 4, T>(cn, bz, cz1, wxy, wz0, wz1, ec, Ntot_c, uf, Ntot_f, gid, done, ndc, ci);
+  else if (nk == 2) prolong_columns<ND, NC, 2, T>(cn, bz, cz1, wxy, wz0, wz1, ec, Ntot_c, uf, Ntot_f, gid, done, ndc, ci);
+  else prolong_columns<ND, NC, 1, T>(cn, bz, cz1, wxy, wz0, wz1, ec, Ntot_c, uf, Ntot_f, gid, done, ndc, ci);
 }
 
 // Launches of 4096 or more blocks (level 0 of a batch of four 64^3 cells) use the node form; smaller ones

@@ -298,3 +298,28 @@ def test_coarsest_inverse(physics):
     assert np.abs(A - H.A[-1].toarray()).max() <= 1e-13 * np.abs(A).max()
     del rt
     cb.close()
+
+
+@pytest.mark.parametrize("physics", ["elastic", "thermal"])
+@pytest.mark.parametrize("dims", [[64, 64, 64, 64], [64, 64, 64, 64, 48, 32]])
+def test_prolong_node_form_matches_oracle(dims, physics):
+    """Batches with at least 4096 level-0 blocks take the node form of the level-0
+    prolongation (one thread per fine node for all cases, one branch on the x/y parity);
+    in the 48^3 and 32^3 cells a wavefront spans two z-pencils and takes both branches."""
+    import maple_amd
+    from oracle import mg
+    print(f"[prolong node form] {physics} {dims}", flush=True)
+    nd, nc = (3, 6) if physics == "elastic" else (1, 3)
+    cb = maple_amd.CellBatch(dims, nlevels=4, device=DEV)
+    assert sum(r ** 3 for r in dims) // 256 >= 4096
+    rng = np.random.default_rng(9)
+    coarse = rng.standard_normal((nc, nd, cb.num_nodes(1)))
+    fine = cb.prolong(physics, 1, torch.from_numpy(coarse).to(DEV)).cpu().numpy()
+    o0 = o1 = 0
+    for r in dims:
+        n0, n1 = r ** 3, (r // 2) ** 3
+        P = mg.prolongation((r, r, r), (r // 2,) * 3, 1)
+        ref = np.stack([np.stack([P @ coarse[j, c, o1:o1 + n1] for c in range(nd)]) for j in range(nc)])
+        np.testing.assert_allclose(fine[:, :, o0:o0 + n0], ref, rtol=1e-14, atol=1e-14)
+        o0, o1 = o0 + n0, o1 + n1
+    cb.close()
