@@ -43,9 +43,9 @@ Forms (one JSON line; ``value`` is the headline form):
 
 Beside the headline (each can be switched off):
 * ``config4``: the 64-cell mixed-resolution batch (16 x 32^3, 32 x 48^3, 16 x 64^3,
-  per-cell seeds, simplex weights and volume fractions), 100 OC iterations, cell
-  iterations / s and 64^3-equivalent iterations / s; solves that stop early are
-  counted, not fatal.
+  per-cell seeds, simplex weights and volume fractions) as one batch whose CG
+  iterations replay as captured hipGraphs, 100 OC iterations, cell iterations / s and
+  64^3-equivalent iterations / s; solves that stop early are counted, not fatal.
 * ``config3``: one 64^3 cell in the mixed form, J = 0.4 E + 0.3 G + 0.3 kappa, 100
   OC iterations; at every iterate C_H, kappa_H and dJ/drho against a tight
   (rtol 1e-8) FP64 solve of the same density (the FP64 path is pinned to the
@@ -231,7 +231,7 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
     return out
 
 
-def run_config4(steps: int, precision: str = "mixed", maxit: int = 3000):
+def run_config4(steps: int, precision: str = "mixed", maxit: int = 3000, graphs: bool = True):
     """The 64-cell mixed-resolution batch (16 x 32^3, 32 x 48^3, 16 x 64^3) as one batch.
     Solves that stop before rtol are counted (strict=False), not fatal."""
     import numpy as np
@@ -247,6 +247,7 @@ def run_config4(steps: int, precision: str = "mixed", maxit: int = 3000):
     cells.set_sweeps(3, 3)
     cells.set_chebyshev(0.1)
     cells.set_start_history(4)
+    cells.set_graphs(graphs)
     from maple_amd import tune as mt
     tuned = {k: v for k, (v, _) in mt.autotune(cells).items()}
     drv = maple_amd.OCDriver(cells, [c["weights"] for c in spec], [c["volfrac"] for c in spec], move=0.2, xmin=1e-3,
@@ -262,12 +263,18 @@ def run_config4(steps: int, precision: str = "mixed", maxit: int = 3000):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     elems = sum(c["r"] ** 3 for c in spec)
+    captures, replays = cells.graph_counts()
     cells.close()
     return {"steps": steps, "ms_per_step": dt * 1e3 / steps, "cells": len(spec), "elements": elems,
             "cell_oc_iters_per_s": len(spec) * steps / dt, "equiv_64cube_oc_iters_per_s": elems / R ** 3 * steps / dt,
             "unconverged_solves": unconv, "max_cg_iters": worst, "tuned_variants": tuned,
-            "form": "one batch, mixed precision, warm start, island correction, coarsest inverse every 3rd density, "
-                    "3+3 Chebyshev-weighted l1-Jacobi sweeps over [0.1, 1]"}
+            "graph_captures": captures, "graph_replays": replays,
+            "form": "one batch, mixed precision, island correction, coarsest inverse every 3rd density, "
+                    "3+3 Chebyshev-weighted l1-Jacobi sweeps over [0.1, 1], solves started on the projection over "
+                    "the last 4 solutions, CG iterations replayed as captured hipGraphs" if graphs else
+                    "one batch, mixed precision, island correction, coarsest inverse every 3rd density, "
+                    "3+3 Chebyshev-weighted l1-Jacobi sweeps over [0.1, 1], solves started on the projection over "
+                    "the last 4 solutions"}
 
 
 def run_config3(steps: int):
