@@ -64,6 +64,8 @@ preconditioner) with HIP events on the stream it runs on.
 from __future__ import annotations
 
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -96,6 +98,19 @@ KERNEL_NAME = {8: "k_fine_nodeb64<{t},*,3> (level-0 elastic operator, vector FMA
                9: "k_fine_mfma<*,4> (level-0 elastic operator, element products on v_mfma_f32_16x16x4f32"}
 
 
+@contextlib.contextmanager
+def _quiet_gc():
+    """Collect garbage once, then keep the collector off for the timed region: an object
+    it frees there (a network or context left over from an earlier run) releases device
+    memory, and hipFree waits for the whole device."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
+
+
 def _x0(ncells):
     import numpy as np
 
@@ -118,6 +133,7 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
     dev = torch.device("cuda", 0)
     per = ncells // groups
     tuned = {}
+    nets = []
 
     def make():
         out = []
@@ -144,7 +160,9 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
             start = None
             if learned:
                 from maple_amd import learned as ml
-                start = ml.LearnedStart(cells, ml.UNet.load(max_batch=min(16, per), device=dev), mode="delta")
+                net = ml.UNet.load(max_batch=min(16, per), device=dev)
+                nets.append(net)
+                start = ml.LearnedStart(cells, net, mode="delta")
             drv = maple_amd.OCDriver(cells, OBJ, 0.25, move=0.2, xmin=1e-3, filter_k=3, rtol=1e-6, maxit=maxit,
                                      start=start)
             out.append((cells, drv))
@@ -158,6 +176,10 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
         torch.cuda.synchronize()
         for c, _ in sets:
             c.close()
+        for net in nets:
+            net.close()
+        nets.clear()
+        del sets
     sets = make()
     kind = 0 if precision == "fp64" else 2
     if profile_kernel:
@@ -170,11 +192,12 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
         iters[g, k] = info["iters"]
 
     co = ConcurrentOC([(c, d, x.clone()) for (c, d), x in zip(sets, xs)])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    res = co.run(steps, on_step)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    with _quiet_gc():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = co.run(steps, on_step)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
     count, kms, work = 0, 0.0, 0
     if profile_kernel:
         for c, _ in sets:
@@ -201,6 +224,8 @@ def run_form(steps: int, warmup: int, ncells: int = 1, precision: str = "fp64", 
     J_last = float(res[0][1]["J"][0])
     for c, _ in sets:
         c.close()
+    for net in nets:
+        net.close()
     dt = t1 - t0
     # per step: the largest count over the cells of a group, averaged over the groups
     it_el = iters[..., :6].max(axis=(2, 3))
@@ -254,14 +279,15 @@ def run_config4(steps: int, precision: str = "mixed", maxit: int = 3000, graphs:
                              filter_k=3, rtol=1e-6, maxit=maxit, physics=("elastic", "thermal"), strict=False)
     x = torch.from_numpy(np.concatenate([c["x0"] for c in spec])).to(dev)
     unconv, worst = 0, 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        x, info = drv.step(x)
-        unconv += info["unconverged"]
-        worst = max(worst, int(info["iters"].max()))
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    with _quiet_gc():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            x, info = drv.step(x)
+            unconv += info["unconverged"]
+            worst = max(worst, int(info["iters"].max()))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
     elems = sum(c["r"] ** 3 for c in spec)
     captures, replays = cells.graph_counts()
     cells.close()
@@ -362,12 +388,13 @@ def run_config1_gpu(steps: int = 50, window=None):
     drv = maple_amd.OCDriver(cells, "bulk:1.0", 0.25, move=0.2, xmin=1e-3, filter_k=3, rtol=1e-6, maxit=5000)
     x = torch.from_numpy(maple_amd.x0_sphere_void(r, seed=0)).to(dev)
     times = []
-    for _ in range(steps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        x, info = drv.step(x)
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t0)
+    with _quiet_gc():
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x, info = drv.step(x)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
     cells.close()
     out = {"oc_iters_per_s": steps / sum(times), "steps": steps, "median_s": float(np.median(times)),
            "J_last": float(info["J"][0])}

@@ -15,6 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from . import streams as _streams
 
 PHYS = {"elastic": L.MAPLE_ELASTIC, "thermal": L.MAPLE_THERMAL}
 NCASE = {"elastic": 6, "thermal": 3}
@@ -61,6 +62,10 @@ class CellBatch:
             raise ValueError("CellBatch needs a CUDA (HIP) device; there is no CPU path")
         prec = {"fp64": L.MAPLE_PREC_FP64, "mixed": L.MAPLE_PREC_MIXED}[precision]
         self.precision = precision
+        # the process's concurrent-work streams come before the context's own streams
+        # (their hardware queues; maple_amd/streams.py)
+        if torch.cuda.is_available():
+            _streams.reserve(self.device)
         flat = (C.c_int * (3 * self.ncells))(*[v for d in dl for v in d])
         h = C.c_void_p()
         L.check(self.lib.maple_ctx_create(self.device.index or 0, self.ncells, flat, self.nlevels, prec, C.byref(h)))

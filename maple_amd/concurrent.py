@@ -26,6 +26,7 @@ import torch
 
 from .cells import CellBatch
 from .optim import OCDriver
+from .streams import group_streams
 
 
 class ConcurrentOC:
@@ -37,7 +38,9 @@ class ConcurrentOC:
 
     def __init__(self, groups: Sequence[Tuple[CellBatch, OCDriver, torch.Tensor]]):
         self.groups = list(groups)
-        self.streams = [torch.cuda.Stream(device=g[0].device) for g in self.groups]
+        if len({g[0].device for g in self.groups}) > 1:
+            raise ValueError("the groups of a ConcurrentOC must share one device")
+        self.streams = group_streams(self.groups[0][0].device, len(self.groups)) if self.groups else []
 
     def run(self, steps: int, on_step: Callable[[int, int, dict], None] = None):
         n = len(self.groups)

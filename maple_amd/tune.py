@@ -15,8 +15,8 @@ kept per batch shape (cell sizes, precision, level count) for the process:
 ``autotune(cells)`` on a second batch of the same shape reuses it.
 
 ``streams > 1`` times every variant as that many concurrent copies on separate
-streams, for batches that will share the device with as many others (concurrent
-groups): a kernel that wins alone can lose there.  The FP32 MFMA operator (variant
+streams (best of two interleaved trials), for batches that will share the device with
+as many others (concurrent groups): a kernel that wins alone can lose there.  The FP32 MFMA operator (variant
 9) is such a case: alone it is faster than the vector node-block form for 4-cell
 64^3 groups, beside three other groups it is slower (its LDS and two-wavefront
 occupancy leave less room for their kernels).
@@ -77,11 +77,13 @@ def autotune(cells: CellBatch, reps: int = 5, seed: int = 0, force: bool = False
     returns {kernel: (variant, {variant: ms})} ({kernel: (variant, None)} from the cache).
     streams > 1: time each variant as that many concurrent copies (one per stream)."""
     key = shape_key(cells, streams)
-    sts = [torch.cuda.Stream(device=cells.device) for _ in range(streams)] if streams > 1 else None
     if not force and key in _CACHE:
         for name, v in _CACHE[key].items():
             cells.set_variant(KERNELS[name][0], v)
         return {name: (v, None) for name, v in _CACHE[key].items()}
+    # streams of its own: timed on the process's group streams (maple_amd/streams.py) the
+    # choice varied from run to run and was up to 13 % slower on the 4-group headline
+    sts = [torch.cuda.Stream(device=cells.device) for _ in range(streams)] if streams > 1 else None
     g = torch.Generator(device=cells.device).manual_seed(seed)
     dev, n = cells.device, cells.nelem
     x = torch.rand(n, generator=g, device=dev, dtype=torch.float64)
@@ -107,9 +109,11 @@ def autotune(cells: CellBatch, reps: int = 5, seed: int = 0, force: bool = False
         if name == "coarse_operator" and not uc:
             continue
         times = {}
-        for v in variants:
-            cells.set_variant(kid, v)
-            times[v] = _time(roles[name], reps, sts)
+        for trial in range(2 if sts else 1):   # concurrent timings are noisier: best of two, interleaved
+            for v in variants:
+                cells.set_variant(kid, v)
+                t = _time(roles[name], reps, sts)
+                times[v] = min(times.get(v, t), t)
         best = min(times, key=times.get)
         cells.set_variant(kid, best)
         out[name] = (best, times)
