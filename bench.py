@@ -329,7 +329,9 @@ def run_config3(steps: int):
     def rel(a, b):
         return float((a - b).abs().max() / b.abs().max())
 
-    for _ in range(steps):
+    for k in range(steps):
+        if k and k % 20 == 0:
+            print(f"# config3: {k} of {steps} iterates checked", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         x_used = x.clamp(1e-3, 1.0)
