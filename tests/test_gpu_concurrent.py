@@ -49,3 +49,24 @@ def test_group_errors_reach_the_caller():
     with pytest.raises(ValueError):
         ConcurrentOC([(c, d, bad)]).run(1)
     c.close()
+
+
+def test_group_streams_are_reserved_once_and_reused():
+    """The first CellBatch of the device reserves the process's concurrent-work streams
+    before its context makes its own; later callers (groups, repeated runs) get those
+    same streams back instead of fresh ones."""
+    from maple_amd import streams
+    from maple_amd.concurrent import ConcurrentOC
+    cb, drv = _make(1)
+    first = streams.group_streams(DEV, streams.RESERVED)
+    assert len(first) == streams.RESERVED
+    assert len({s.cuda_stream for s in first}) == streams.RESERVED       # distinct streams
+    again = streams.group_streams(DEV, 3)
+    assert [s.cuda_stream for s in again] == [s.cuda_stream for s in first[:3]]
+    x = torch.full((cb.nelem,), 0.25, dtype=torch.float64, device=DEV)
+    co1 = ConcurrentOC([(cb, drv, x.clone())])
+    co2 = ConcurrentOC([(cb, drv, x.clone())])
+    assert co1.streams[0].cuda_stream == co2.streams[0].cuda_stream == first[0].cuda_stream
+    more = streams.group_streams(DEV, streams.RESERVED + 2)              # grows on demand, keeps the first ones
+    assert [s.cuda_stream for s in more[:streams.RESERVED]] == [s.cuda_stream for s in first]
+    cb.close()
