@@ -16,10 +16,13 @@ kept per batch shape (cell sizes, precision, level count) for the process:
 
 ``streams > 1`` times every variant as that many concurrent copies on separate
 streams (best of two interleaved trials), for batches that will share the device with
-as many others (concurrent groups): a kernel that wins alone can lose there.  The FP32 MFMA operator (variant
-9) is such a case: alone it is faster than the vector node-block form for 4-cell
-64^3 groups, beside three other groups it is slower (its LDS and two-wavefront
-occupancy leave less room for their kernels).
+as many others (concurrent groups): a kernel that wins alone can lose there.  Such
+timings scatter by a few per cent, so a candidate replaces the library's default only
+when it is faster by more than CONCURRENT_MARGIN.  The FP32 MFMA operator (variant 9)
+is the case in point: alone it is a little faster than the vector node-block form for
+4-cell 64^3 groups; timed as four concurrent copies the two lie within 2 %, and in the
+16-cell headline (four such groups beside each other's coarse-level and vector
+kernels) variant 9 runs 92-94 cell it/s against 97.5-100.4 for variant 8.
 """
 from __future__ import annotations
 
@@ -36,6 +39,11 @@ KERNELS = {
     "thermal_operator": (L.MAPLE_KERNEL_THERMAL_OPERATOR, (0, 1, 2, 3)),
     "coarse_operator": (L.MAPLE_KERNEL_COARSE_OPERATOR, (0, 1, 2)),
 }
+
+# the library's default variant of each kernel (the fine operator's 8 becomes 5 in a
+# batch with a cell whose nz is not 64)
+DEFAULTS = {"fine_operator": 8, "thermal_operator": 3, "coarse_operator": 2}
+CONCURRENT_MARGIN = 0.05
 
 _CACHE: Dict[Tuple, Dict[str, int]] = {}
 
@@ -115,6 +123,9 @@ def autotune(cells: CellBatch, reps: int = 5, seed: int = 0, force: bool = False
                 t = _time(roles[name], reps, sts)
                 times[v] = min(times.get(v, t), t)
         best = min(times, key=times.get)
+        d = DEFAULTS.get(name)
+        if sts and d in times and times[best] > (1.0 - CONCURRENT_MARGIN) * times[d]:
+            best = d
         cells.set_variant(kid, best)
         out[name] = (best, times)
         chosen[name] = best

@@ -4,7 +4,8 @@ It must set, for every tunable kernel, one of that kernel's variants and the one
 own timings rank fastest; the solve with the tuned variants must give the results of
 the default ones (every variant computes the same operator: parity tests pin each one
 against the oracle); a second call on a batch of the same shape reuses the choice; the
-concurrent form (streams > 1) obeys the same rules.
+concurrent form (streams > 1) keeps the library's default unless a candidate beats it
+by the margin.
 """
 import numpy as np
 import pytest
@@ -44,7 +45,14 @@ def test_autotune_choice_and_results(precision, dims):
     assert all(t is None for _, t in again.values())
     conc = mt.autotune(cells, reps=2, force=True, streams=3)
     for name, (v, times) in conc.items():
-        assert v in mt.KERNELS[name][1] and times[v] == min(times.values())
+        fastest = min(times.values())
+        assert v in mt.KERNELS[name][1]
+        # concurrent timings: the default stays unless a candidate beats it by the margin
+        d = mt.DEFAULTS[name]
+        if times[d] <= fastest / (1.0 - mt.CONCURRENT_MARGIN):
+            assert v == d, (name, v, times)
+        else:
+            assert times[v] == fastest, (name, v, times)
     CH2, _, _ = cells.homogenize(rho, rtol=1e-9, maxit=2000)
     assert float(torch.linalg.norm(CH2 - CH0) / torch.linalg.norm(CH0)) <= 1e-7
     cells.close()
