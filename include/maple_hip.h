@@ -175,8 +175,9 @@ int maple_set_coarse_refresh(maple_ctx* ctx, int every);
  * error, (u* - u0)^T K (u* - u0): c solves the m x m (or m+1) Galerkin system
  * (h_a^T K h_b) c = h_a^T f, leaving out vectors that are numerically combinations
  * of newer ones, so the previous fields (warm start) and the supplied start are
- * special cases.  Each solve costs one operator application per basis vector; the
- * context holds 2m+1 copies of each physics' fields.  Solves by the per-case CG
+ * special cases (with a history the previous fields are always in the basis, whatever
+ * maple_set_solver's warm_start says).  Each solve costs one operator application per
+ * basis vector; the context holds 2m+1 copies of each physics' fields.  Solves by the per-case CG
  * only (not block CG).  Changing m clears the history.  No counterpart in the
  * reference, which starts every solve from zero (EBE_GMG.py:852-858). */
 int maple_set_start_history(maple_ctx* ctx, int m);
