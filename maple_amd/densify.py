@@ -86,7 +86,8 @@ class DensifyBatch:
     """Tasks of one resolution advanced together (one CellBatch; one launch set per OC step)."""
 
     def __init__(self, tasks: Sequence[Task], device=None, precision: str = "mixed", rtol: float = 1e-6,
-                 maxit: int = 5000, nlevels: Optional[int] = None, islands: bool = True, **settings):
+                 maxit: int = 5000, nlevels: Optional[int] = None, islands: bool = True, start_history: int = 4,
+                 **settings):
         if not tasks:
             raise ValueError("no tasks")
         self.s = dict(DENSIFY_DEFAULTS)
@@ -105,6 +106,8 @@ class DensifyBatch:
         n = len(self.tasks)
         self.cells = CellBatch([self.r] * n, nlevels=nlevels or max_levels(self.r), device=self.device,
                                precision=precision, warm_start=True, islands=islands)
+        # each solve starts from the energy-optimal combination of the task's last solutions
+        self.cells.set_start_history(start_history)
         self.rtol, self.maxit = float(rtol), int(maxit)
         self.weights = [parse_objective_spec(t.objective) for t in self.tasks]
         self.kind, self.axes = parse_symmetry(self.s["symmetry"])
