@@ -15,8 +15,9 @@ total derivative <dJ/dC_H, S_e> (converged sensitivity, main_IHD_v3.py:688)
 for rho and None for F, which already depends on rho through
 assemble_F_fast (model.py:98-159 semantics; F is re-assembled on the device
 from rho, so the F argument only has to have the reference's shape).
-Precision, rtol and warm start are constructor keywords; the positional
-signature is unchanged.  Results are float64.
+Precision, rtol, warm start and the start history (each solve started from the
+energy-optimal combination of the last solutions) are constructor keywords; the
+positional signature is unchanged.  Results are float64.
 """
 from __future__ import annotations
 
@@ -68,7 +69,8 @@ class _Homogenize(torch.autograd.Function):
 class IHD(nn.Module):
     def __init__(self, cfg, device, *, rtol: float = 1e-6, maxit: int = 1000, nlevels: Optional[int] = None,
                  precision: str = "fp64", warm_start: bool = False, omega: Optional[float] = None, nsmooth: int = 2,
-                 E0: float = 1.0, nu: float = 0.3, kappa0: float = 1.0, islands: bool = False):
+                 E0: float = 1.0, nu: float = 0.3, kappa0: float = 1.0, islands: bool = False,
+                 start_history: int = 0):
         super().__init__()
         self.cfg = cfg
         self.r = self.resolution = int(cfg.resolution)
@@ -95,6 +97,9 @@ class IHD(nn.Module):
                                omega=omega, nsmooth=nsmooth, warm_start=warm_start, islands=islands)
         self._precision, self._omega, self._nsmooth = precision, omega, nsmooth
         self._warm, self._islands = warm_start, islands
+        self._history = int(start_history)
+        if self._history:
+            self.cells.set_start_history(self._history)
         self._batches = {}
         self._custom = None
         if (E0, nu, kappa0) != (1.0, 0.3, 1.0):
@@ -114,6 +119,8 @@ class IHD(nn.Module):
             for i in range(nb):
                 if self._custom is not None:
                     cb.set_element_matrices(i, *self._custom)
+            if self._history:
+                cb.set_start_history(self._history)
             self._batches[nb] = cb
         return cb
 

@@ -155,3 +155,39 @@ def test_trajectory_with_history(case):
             assert np.linalg.norm(a[1][c] - b[1][c]) <= 1e-6 * np.linalg.norm(a[1][c]), f"step {k} cell {c}"
         assert float((a[2] - b[2]).abs().max()) <= 1e-5, f"step {k}: designs differ"
     assert it3 < it0, (it3, it0)
+
+
+def test_ihd_dropin_with_history():
+    """The reference-shaped IHD with start_history: a sequence of nearby designs gives the
+    C_H and kappa_H of the drop-in without it (both at rtol 1e-10), and from the second
+    call on the solves start projected on the earlier solutions."""
+    import maple_amd
+    r = 16
+    cfg = maple_amd.make_cfg(r, smooth_iter=[2, 2, 3])
+    plain = maple_amd.IHD(cfg, DEV, rtol=1e-10, maxit=4000)
+    hist = maple_amd.IHD(cfg, DEV, rtol=1e-10, maxit=4000, start_history=3)
+    rng = np.random.default_rng(8)
+    x = np.clip(0.3 + rng.uniform(-0.2, 0.2, r ** 3), 1e-3, 1.0)
+    it_plain = it_hist = 0
+    for k in range(4):
+        x = np.clip(x + 0.02 * rng.standard_normal(r ** 3), 1e-3, 1.0)
+        out = []
+        for ihd in (plain, hist):
+            batch = maple_amd.build_unitcell_batch(DEV, r, torch.from_numpy(x).reshape(r, r, r).float())
+            rho = torch.from_numpy(1e-9 + (1 - 1e-9) * x ** 3).reshape(-1, 1).to(DEV)
+            batch["rho"] = rho
+            n = batch["feature"].shape[0]
+            CH = ihd.TO_homo_3d(batch, ihd.Ke_3d, maple_amd.assemble_F_fast(batch["node_index"], ihd.Fe_3d, n, rho),
+                                ihd.X0_3d, batch_size=1)
+            its = int(ihd.cells.last_iters[0, :6].sum())
+            kap = ihd.TO_homo_1d(batch, ihd.Ke_1d,
+                                 maple_amd.assemble_F_fast_scalar(batch["node_index"], ihd.Fe_1d, n, rho),
+                                 ihd.X0_1d, batch_size=1)
+            out.append((CH.detach(), kap.detach(), its))
+        (CHp, kp, ip), (CHh, kh, ih) = out
+        assert float(torch.linalg.norm(CHh - CHp) / torch.linalg.norm(CHp)) <= 1e-8, k
+        assert float(torch.linalg.norm(kh - kp) / torch.linalg.norm(kp)) <= 1e-8, k
+        if k:
+            assert (hist.cells.start_sources("elastic") == 3).all()
+            it_plain, it_hist = it_plain + ip, it_hist + ih
+    assert it_hist < it_plain, (it_hist, it_plain)
