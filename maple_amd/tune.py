@@ -27,7 +27,7 @@ kernels) variant 9 runs 92-94 cell it/s against 97.5-100.4 for variant 8.
 from __future__ import annotations
 
 import time
-from typing import Dict, Tuple
+from typing import Dict, Optional, Tuple
 
 import torch
 
@@ -76,6 +76,15 @@ def _time(fn, reps: int, streams=None) -> float:
     return (time.perf_counter() - t0) * 1e3 / reps
 
 
+def choose(times: Dict[int, float], default: Optional[int], concurrent: bool) -> int:
+    """The variant to keep: the fastest; under concurrent timing the default unless the
+    fastest beats it by more than CONCURRENT_MARGIN."""
+    best = min(times, key=times.get)
+    if concurrent and default in times and times[best] > (1.0 - CONCURRENT_MARGIN) * times[default]:
+        return default
+    return best
+
+
 def shape_key(cells: CellBatch, streams: int = 1) -> Tuple:
     return (tuple(cells.dims), cells.precision, cells.nlevels, int(streams))
 
@@ -122,10 +131,7 @@ def autotune(cells: CellBatch, reps: int = 5, seed: int = 0, force: bool = False
                 cells.set_variant(kid, v)
                 t = _time(roles[name], reps, sts)
                 times[v] = min(times.get(v, t), t)
-        best = min(times, key=times.get)
-        d = DEFAULTS.get(name)
-        if sts and d in times and times[best] > (1.0 - CONCURRENT_MARGIN) * times[d]:
-            best = d
+        best = choose(times, DEFAULTS.get(name), sts is not None)
         cells.set_variant(kid, best)
         out[name] = (best, times)
         chosen[name] = best

@@ -142,3 +142,20 @@ def test_warmstart_network_layout_matches_oracle(lib):
     f = learned.density_feature(rho, 8, 4).numpy()
     ref = (rho.numpy() ** (1 / 3)).reshape(4, 2, 4, 2, 4, 2).mean(axis=(1, 3, 5)).ravel()
     np.testing.assert_allclose(f, ref, rtol=1e-6)
+
+
+def test_tuner_choice_rule():
+    """The tuner keeps the fastest variant; under concurrent timing (which scatters by a
+    few per cent) the library's default unless a candidate beats it by the margin."""
+    from maple_amd import tune
+    m = tune.CONCURRENT_MARGIN
+    t = {5: 1.00, 8: 1.02, 9: 0.99}
+    assert tune.choose(t, 8, concurrent=False) == 9
+    assert tune.choose(t, 8, concurrent=True) == 8                 # 9 is within the margin of 8
+    t = {5: 1.00, 8: 1.02, 9: 1.02 * (1 - m) - 1e-3}
+    assert tune.choose(t, 8, concurrent=True) == 9                 # clearly faster: taken
+    assert tune.choose({0: 2.0, 1: 1.0}, None, concurrent=True) == 1   # no default: the fastest
+    assert tune.choose({0: 2.0, 1: 1.0}, 7, concurrent=True) == 1      # default not timed: the fastest
+    assert set(tune.DEFAULTS) == set(tune.KERNELS)
+    for name, v in tune.DEFAULTS.items():
+        assert v in tune.KERNELS[name][1]
