@@ -155,6 +155,10 @@ def test_trajectory_with_history(case):
             assert np.linalg.norm(a[1][c] - b[1][c]) <= 1e-6 * np.linalg.norm(a[1][c]), f"step {k} cell {c}"
         assert float((a[2] - b[2]).abs().max()) <= 1e-5, f"step {k}: designs differ"
     assert it3 < it0, (it3, it0)
+    again, it3b = _trajectory(3, steps=12, **case)        # fixed-order reductions: bit for bit
+    assert it3b == it3
+    for k, (a, b) in enumerate(zip(proj, again)):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and torch.equal(a[2], b[2]), k
 
 
 def test_ihd_dropin_with_history():
