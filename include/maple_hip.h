@@ -178,7 +178,7 @@ int maple_set_coarse_refresh(maple_ctx* ctx, int every);
  * special cases (with a history the previous fields are always in the basis, whatever
  * maple_set_solver's warm_start says).  Each solve costs one operator application per
  * basis vector; the context holds 2m+1 copies of each physics' fields.  Solves by the per-case CG
- * only (not block CG).  Changing m clears the history.  No counterpart in the
+ * only (not block CG).  Every call clears the history.  No counterpart in the
  * reference, which starts every solve from zero (EBE_GMG.py:852-858). */
 int maple_set_start_history(maple_ctx* ctx, int m);
 

@@ -674,7 +674,8 @@ __global__ __launch_bounds__(64) void k_proj_coef(const LevelCell* __restrict__ 
 }
 
 // us = sum_a c_a h_a, rs = f - sum_a c_a g_a, partial (rs, rs) per case.  us may be one
-// of the basis vectors (each thread reads its node's values before writing them).
+// of the basis vectors (each thread reads its node's values before writing them).  A
+// vector left out (c_a = 0) is not read: the fields of a failed solve may hold NaN.
 template <int ND>
 __global__ __launch_bounds__(256) void k_proj_combine(Launch L, int ncase, int k, ProjBasis B,
                                                       const double* __restrict__ coef, const double* __restrict__ f,
@@ -697,7 +698,7 @@ __global__ __launch_bounds__(256) void k_proj_combine(Launch L, int ncase, int k
         double u = 0.0, r = f[idx];
 #pragma unroll
         for (int a = 0; a < MAPLE_PROJ_MAX; ++a)
-          if (a < k) {
+          if (a < k && c[a] != 0.0) {
             u += c[a] * B.h[a][idx];
             r -= c[a] * B.g[a][idx];
           }
