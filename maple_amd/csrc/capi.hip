@@ -88,6 +88,7 @@ struct Phys {
   double *Ke = nullptr, *Fe = nullptr, *X0 = nullptr, *G = nullptr;   // per cell on device
   double* lap = nullptr;   // thermal: Q1-Laplacian coefficients k[0..3] per cell (node-form operator)
   bool laplacian = false;  // thermal: every cell's Ke is a Q1 Laplacian (k[Hamming distance])
+  bool face0 = false;      // and its distance-1 entry is zero (to 1e-14 of the largest entry)
   std::vector<double> hKe, hFe, hX0;                                   // per cell on host
   VC<double> v64;
   VC<float> v32;
@@ -272,7 +273,7 @@ int upload_phys_constants(maple_ctx* c, Phys& P) {
   if (P.nd == 1) {
     // node-form thermal operator: Ke[a][b] = k[d(a, b)], d the Hamming distance of corners a, b
     std::vector<double> lap((size_t)c->ncells * 4, 0.0);
-    bool ok = true;
+    bool ok = true, f0 = true;
     for (int i = 0; i < c->ncells && ok; ++i) {
       const double* K = &P.hKe[(size_t)i * 64];
       double* k = &lap[(size_t)i * 4];
@@ -289,8 +290,10 @@ int upload_phys_constants(maple_ctx* c, Phys& P) {
             ok = false;
           }
         }
+      if (std::fabs(k[1]) > 1e-14 * scale) f0 = false;
     }
     P.laplacian = ok;
+    P.face0 = ok && f0;
     HIPCHK(hipMemcpy(P.lap, lap.data(), lap.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   return 0;
@@ -511,7 +514,7 @@ void apply_fine64(maple_ctx* c, Phys& P, int mode, const double* rho, const doub
   else if (P.nd == 3)
     fine_apply<3, double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
   else if (L.var >= 2)
-    thermal_apply64<double>(L, mode, P.nc, P.lap, rho, u, f, dinv, c->omega, out, partial, s);
+    thermal_apply64<double>(L, mode, P.nc, P.lap, rho, u, f, dinv, c->omega, out, partial, s, P.face0);
   else
     fine_apply<1, double>(L, mode, P.nc, P.Ke, rho, u, f, dinv, c->omega, out, partial, s);
   timed_end(c, kind, s);
@@ -543,7 +546,7 @@ void apply_t(maple_ctx* c, Phys& P, VC<T>& V, int l, int mode, const T* u, const
     else if (P.nd == 3)
       fine_apply<3, T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], om, out, nullptr, s);
     else if (L.var >= 2)
-      thermal_apply64<T>(L, mode, P.nc, P.lap, V.rho, u, f, V.dinv[0], om, out, nullptr, s);
+      thermal_apply64<T>(L, mode, P.nc, P.lap, V.rho, u, f, V.dinv[0], om, out, nullptr, s, P.face0);
     else
       fine_apply<1, T>(L, mode, P.nc, V.Ke, V.rho, u, f, V.dinv[0], om, out, nullptr, s);
     if (kind >= 0) timed_end(c, kind, s);

@@ -45,10 +45,12 @@ __device__ __forceinline__ T s_sum(const T (&r)[2][2][2], int dx, int dy, int dz
 }  // namespace
 
 // MODE: 0 = K u, 1 = K u and partial (u, K u), 2 = f - K u, 3 = u + omega Dinv (f - K u).
+// F0: the coefficient of Hamming distance 1 is zero (the Q1 Laplacian's: h/12 [4, 0, -1, -1]),
+// so the six face neighbours carry no coupling and are skipped (21 of the 27 per node).
 // Thread t of a cell owns the NX nodes (NX gx + k, y, z), t = (gx*ny + y)*64 + z: NX = 2 on
 // the pair launch of the level-0 grid (neighbour columns shared by two nodes), NX = 1 on
 // the node launch (half the live registers, twice the waves in flight).
-template <typename T, int MODE, int NX>
+template <typename T, int MODE, int NX, bool F0>
 __global__ __launch_bounds__(256) void k_thermal64(
     const LevelCell* __restrict__ cells, const int* __restrict__ blk_start, int ncells, long long Ntot,
     int ncase, const double* __restrict__ coef, const T* __restrict__ rho, const T* __restrict__ u,
@@ -131,9 +133,11 @@ __global__ __launch_bounds__(256) void k_thermal64(
           const int dx = ix - 1 - kx;
           if (dx < -1 || dx > 1) continue;
           if (dx == 0 && dy == 0) uown[kx][j] = uc;
-          acc[kx][j] = fma(w[kx][dx + 1][dy + 1][0], um, acc[kx][j]);
-          acc[kx][j] = fma(w[kx][dx + 1][dy + 1][1], uc, acc[kx][j]);
-          acc[kx][j] = fma(w[kx][dx + 1][dy + 1][2], up, acc[kx][j]);
+          const bool zface = F0 && dx == 0 && dy == 0;               // (0, 0, -1), (0, 0, +1)
+          const bool xyface = F0 && (dx != 0) + (dy != 0) == 1;      // (+-1, 0, 0), (0, +-1, 0)
+          if (!zface) acc[kx][j] = fma(w[kx][dx + 1][dy + 1][0], um, acc[kx][j]);
+          if (!xyface) acc[kx][j] = fma(w[kx][dx + 1][dy + 1][1], uc, acc[kx][j]);
+          if (!zface) acc[kx][j] = fma(w[kx][dx + 1][dy + 1][2], up, acc[kx][j]);
         }
       }
   }
@@ -165,23 +169,28 @@ __global__ __launch_bounds__(256) void k_thermal64(
   if constexpr (MODE == 1) block_sum<NC>(dj, partial + (long long)lbid() * MAPLE_MAX_CASES);
 }
 
-template <typename T, int NX>
+template <typename T, int NX, bool F0>
 void thermal_apply64_g(const Launch& L, int mode, int ncase, const double* coef, const T* rho, const T* u,
                        const T* f, const T* dinv, T omega, T* out, double* partial, hipStream_t s) {
   switch (mode) {
-    case 0: k_thermal64<T, 0, NX><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
-    case 1: k_thermal64<T, 1, NX><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
-    case 2: k_thermal64<T, 2, NX><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
-    default: k_thermal64<T, 3, NX><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    case 0: k_thermal64<T, 0, NX, F0><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    case 1: k_thermal64<T, 1, NX, F0><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    case 2: k_thermal64<T, 2, NX, F0><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
+    default: k_thermal64<T, 3, NX, F0><<<L.nblocks, 256, 0, s>>>(L.cells, L.blk_start, L.ncells, L.Ntot, ncase, coef, rho, u, f, dinv, omega, out, partial, L.done, L.ndc, L.work); break;
   }
 }
 
 // L.var 2: two nodes per thread (pair launch); 3: one node per thread (node launch)
 template <typename T>
 void thermal_apply64(const Launch& L, int mode, int ncase, const double* coef, const T* rho, const T* u, const T* f,
-                     const T* dinv, T omega, T* out, double* partial, hipStream_t s) {
-  if (L.var == 3) thermal_apply64_g<T, 1>(L, mode, ncase, coef, rho, u, f, dinv, omega, out, partial, s);
-  else thermal_apply64_g<T, 2>(L, mode, ncase, coef, rho, u, f, dinv, omega, out, partial, s);
+                     const T* dinv, T omega, T* out, double* partial, hipStream_t s, bool face0) {
+  if (L.var == 3) {
+    if (face0) thermal_apply64_g<T, 1, true>(L, mode, ncase, coef, rho, u, f, dinv, omega, out, partial, s);
+    else thermal_apply64_g<T, 1, false>(L, mode, ncase, coef, rho, u, f, dinv, omega, out, partial, s);
+  } else {
+    if (face0) thermal_apply64_g<T, 2, true>(L, mode, ncase, coef, rho, u, f, dinv, omega, out, partial, s);
+    else thermal_apply64_g<T, 2, false>(L, mode, ncase, coef, rho, u, f, dinv, omega, out, partial, s);
+  }
 }
-template void thermal_apply64<double>(const Launch&, int, int, const double*, const double*, const double*, const double*, const double*, double, double*, double*, hipStream_t);
-template void thermal_apply64<float>(const Launch&, int, int, const double*, const float*, const float*, const float*, const float*, float, float*, double*, hipStream_t);
+template void thermal_apply64<double>(const Launch&, int, int, const double*, const double*, const double*, const double*, const double*, double, double*, double*, hipStream_t, bool);
+template void thermal_apply64<float>(const Launch&, int, int, const double*, const float*, const float*, const float*, const float*, float, float*, double*, hipStream_t, bool);

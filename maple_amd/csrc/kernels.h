@@ -78,10 +78,11 @@ void fine_apply_mfma(const Launch& L, int mode, int ncase, const float* Ke, cons
                      const float* f, const float* dinv, float omega, float* out, hipStream_t s);
 // Thermal operator in node form for nz = 64 cells whose element matrix is the Q1
 // Laplacian (entry k[d] for corners at Hamming distance d; coef: 4 per cell), same
-// modes, all (<= 3) load cases per thread; L: the pair launch.
+// modes, all (<= 3) load cases per thread; L: the pair launch.  face0: k[1] = 0 in every
+// cell (the face neighbours couple nothing and are skipped).
 template <typename T>
 void thermal_apply64(const Launch& L, int mode, int ncase, const double* coef, const T* rho, const T* u, const T* f,
-                     const T* dinv, T omega, T* out, double* partial, hipStream_t s);
+                     const T* dinv, T omega, T* out, double* partial, hipStream_t s, bool face0);
 void fine_rhs(const Launch& L, int nd, int ncase, const double* Fe, const double* rho, double* F, hipStream_t s);
 template <typename T>
 void fine_dinv(const Launch& L, int nd, bool l1, const double* Ke, const double* rho, T* dinv, hipStream_t s);
