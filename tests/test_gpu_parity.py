@@ -455,3 +455,32 @@ def test_thermal_operator_general_element_matrix():
         ref = ogrid.ebe_matvec(u[j].T.copy(), rho, Ka, (r, r, r)).T
         assert np.abs(v[j] - ref).max() <= 1e-13 * np.abs(ref).max()
     cb.close()
+
+
+@pytest.mark.parametrize("variant", [2, 3])
+def test_thermal_node_form_with_face_couplings(variant):
+    """A conductivity matrix with the Hamming-distance structure whose distance-1 entry is
+    not zero (the Q1 Laplacian plus delta * g[d], g = [3, -1, 0, 0]: symmetric, zero row
+    sums) keeps the node form, now with the six face neighbours coupled, and stays exact."""
+    from maple_amd import _lib as L
+    print(f"[thermal node form, face couplings] variant {variant}", flush=True)
+    r = 64
+    cb = _cells([r], 4)
+    Ke, Fe, X0 = ofe.hexahedron_scalar(r)
+    corners = [tuple(int(v) for v in c) for c in ofe.HEX8]   # the element's corner order
+    g = [3.0, -1.0, 0.0, 0.0]
+    delta = 0.05 * Ke[0, 0]
+    Ka = Ke.copy()
+    for a in range(8):
+        for b in range(8):
+            d = sum(int(p != q) for p, q in zip(corners[a], corners[b]))
+            Ka[a, b] += delta * g[d]
+    cb.set_element_matrices(0, Ke8=Ka, Fe8=Fe, X0_8=X0)
+    cb.set_variant(L.MAPLE_KERNEL_THERMAL_OPERATOR, variant)
+    rho = _rho(r, 4)
+    u = np.random.default_rng(6).standard_normal((3, 1, r ** 3))
+    v = cb.apply_operator("thermal", _t(rho), _t(u)).cpu().numpy()
+    for j in range(3):
+        ref = ogrid.ebe_matvec(u[j].T.copy(), rho, Ka, (r, r, r)).T
+        assert np.abs(v[j] - ref).max() <= 1e-13 * np.abs(ref).max()
+    cb.close()
