@@ -70,3 +70,39 @@ def test_group_streams_are_reserved_once_and_reused():
     more = streams.group_streams(DEV, streams.RESERVED + 2)              # grows on demand, keeps the first ones
     assert [s.cuda_stream for s in more[:streams.RESERVED]] == [s.cuda_stream for s in first]
     cb.close()
+
+
+def test_headline_composition_small():
+    """The headline's composition on a small workload: cells in concurrent groups, mixed
+    precision, island correction, Chebyshev-weighted 3+3 sweeps, coarsest inverse every third
+    density, the learned start (delta mode) inside a 4-solution projected start.  After a few
+    OC steps every cell's C_H and kappa_H match a tight FP64 solve of its density to 1e-5,
+    and the solves started from projections."""
+    import maple_amd
+    from maple_amd import learned as ml
+    from maple_amd.concurrent import ConcurrentOC
+    r, per, groups, steps = 32, 2, 2, 5
+    sets = []
+    for g in range(groups):
+        cb = maple_amd.CellBatch([r] * per, nlevels=3, device=DEV, precision="mixed", warm_start=True, islands=True)
+        cb.set_coarse_refresh(3)
+        cb.set_sweeps(3, 3)
+        cb.set_chebyshev(0.1)
+        cb.set_start_history(4)
+        start = ml.LearnedStart(cb, ml.UNet.load(max_batch=per, device=DEV), mode="delta")
+        drv = maple_amd.OCDriver(cb, "stiffness:0.5,thermal:0.5", 0.25, move=0.2, xmin=1e-3, filter_k=3, rtol=1e-6,
+                                 maxit=3000, start=start)
+        x0 = np.concatenate([maple_amd.x0_uniform_noise(r, seed=g * per + k) for k in range(per)])
+        sets.append((cb, drv, torch.from_numpy(x0).to(DEV), start))
+    res = ConcurrentOC([(cb, drv, x) for cb, drv, x, _ in sets]).run(steps)
+    for (cb, _, _, start), (x, info, _) in zip(sets, res):
+        assert (cb.start_sources("elastic") == 3).all() and (cb.start_sources("thermal") == 3).all()
+        ref = maple_amd.CellBatch([r] * per, nlevels=3, device=DEV)
+        CHt, kt, _ = ref.homogenize(info["rho"], rtol=1e-10, maxit=4000)
+        for c in range(per):
+            CH, kap = torch.from_numpy(info["CH"][c]).to(DEV), torch.from_numpy(info["kappa"][c]).to(DEV)
+            assert float(torch.linalg.norm(CH - CHt[c]) / torch.linalg.norm(CHt[c])) <= 1e-5
+            assert float(torch.linalg.norm(kap - kt[c]) / torch.linalg.norm(kt[c])) <= 1e-5
+        ref.close()
+        start.net.close()
+        cb.close()
